@@ -1,0 +1,201 @@
+// Spectral Poisson solve as dense transform products (density.rs:96-166).
+//
+// The reference's three 2-D transforms (DCT-II forward, then DST-III/DCT-III
+// and DCT-III/DST-III inverses, each rows then columns) are six dense
+// products with constant cosine/sine matrices.  Every product here is
+// C = A * Bt^T with A (M x K) and Bt (N x K) both row-major, so both MFMA
+// operands are K-contiguous in LDS; a pass whose output feeds the next
+// pass's right operand stores it transposed.
+//
+// FAST engine: BF16 MFMA (v_mfma_f32_32x32x16_bf16) with split-precision
+// operands x = hi + lo, three products hi*hi + hi*lo + lo*hi accumulated in
+// FP32 (the lo*lo term is below FP32 rounding).  Data operands are split on
+// the fly while staging into LDS; the constant matrices are pre-split.
+// FP64 engine: an LDS-tiled FMA kernel (also the fallback for shapes that
+// are not multiples of the MFMA tile).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "ep_internal.h"
+
+namespace ep {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline int find_job(const GemmJob* jobs, int nj, int blk) {
+    int lo = 0, hi = nj;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (jobs[mid].tile_start <= blk) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Store one output element; kEpiPoisson applies the spectral scaling and
+// the Poisson solve of density.rs:113-153 to coefficient (ky = row, kx = col).
+template <typename T>
+__device__ inline void epi_store(const GemmJob& J, const DesignDev& D, int row, int col, double v) {
+    if (row >= J.M || col >= J.N) return;
+    if (J.epi == kEpiStore) {
+        ((T*)J.C)[(size_t)row * J.N + col] = (T)v;
+    } else if (J.epi == kEpiStoreT) {
+        ((T*)J.C)[(size_t)col * J.M + row] = (T)v;
+    } else {
+        const T* sx = (const T*)D.spec_x;  // [kx] -> (w, 2(1 - cos w))
+        const T* sy = (const T*)D.spec_y;
+        double s = 4.0 / ((double)D.Bx * (double)D.By);
+        if (col == 0) s *= 0.5;
+        if (row == 0) s *= 0.5;
+        const double a = v * s;
+        const double wx = (double)sx[2 * col], lx = (double)sx[2 * col + 1];
+        const double wy = (double)sy[2 * row], ly = (double)sy[2 * row + 1];
+        const double den = lx + ly;
+        double ex = 0.0, ey = 0.0;
+        if (!(den < 1e-30)) {
+            const double phi = a / den;
+            ex = phi * wx;
+            ey = phi * wy;
+        }
+        ((T*)J.C)[(size_t)row * J.N + col] = (T)ex;
+        ((T*)J.C2)[(size_t)row * J.N + col] = (T)ey;
+    }
+}
+
+// --------------------------------------------------------------- FMA -----
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_fma(const GemmJob* jobs, int nj, const DesignDev* dd) {
+    __shared__ T sA[16][64 + 1];
+    __shared__ T sB[16][64 + 1];
+    const int j = find_job(jobs, nj, blockIdx.x);
+    const GemmJob& J = jobs[j];
+    const DesignDev& D = dd[J.design];
+    if (D.st->done) return;
+    const int t = blockIdx.x - J.tile_start;
+    const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
+    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    const T* A = (const T*)J.A;
+    const T* B = (const T*)J.B;
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[i][q] = T(0);
+    for (int k0 = 0; k0 < J.K; k0 += 16) {
+        for (int e = tid; e < 64 * 16; e += 256) {
+            const int r = e >> 4, kk = e & 15;
+            const int k = k0 + kk;
+            sA[kk][r] = (m0 + r < J.M && k < J.K) ? A[(size_t)(m0 + r) * J.K + k] : T(0);
+            sB[kk][r] = (n0 + r < J.N && k < J.K) ? B[(size_t)(n0 + r) * J.K + k] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            T a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = sA[kk][ty + 16 * i];
+#pragma unroll
+            for (int q = 0; q < 4; q++) bb[q] = sB[kk][tx + 16 * q];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[i][q] += a[i] * bb[q];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) epi_store<T>(J, D, m0 + ty + 16 * i, n0 + tx + 16 * q, (double)acc[i][q]);
+}
+
+// -------------------------------------------------------------- MFMA -----
+
+constexpr int kTK = 32;        // K per LDS stage
+constexpr int kLdk = kTK + 8;  // padded row (bf16 elements): 80 bytes
+
+__device__ inline void split8(const float* v, bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        __bf16 h = (__bf16)v[q];
+        hi[q] = h;
+        lo[q] = (__bf16)(v[q] - (float)h);
+    }
+}
+
+// Stage a 64 x 32 operand tile (rows r0.., k0..) into LDS hi/lo planes.
+__device__ inline void stage_tile(__bf16 (*s)[64][kLdk], int kind, const void* p, const void* plo,
+                                  int r0, int k0, int K) {
+    const int tid = threadIdx.x;
+    const int row = tid >> 2, kc = (tid & 3) * 8;
+    const size_t off = (size_t)(r0 + row) * K + k0 + kc;
+    bf16x8 hi, lo;
+    if (kind == 0) {
+        const float4* src = (const float4*)((const float*)p + off);
+        float4 a = src[0], b = src[1];
+        float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        split8(v, hi, lo);
+    } else {
+        hi = *(const bf16x8*)((const __bf16*)p + off);
+        lo = *(const bf16x8*)((const __bf16*)plo + off);
+    }
+    *(bf16x8*)&s[0][row][kc] = hi;
+    *(bf16x8*)&s[1][row][kc] = lo;
+}
+
+// 64 x 64 output tile per block, four waves each owning a 32 x 32 quadrant.
+// Requires M, N multiples of 64 and K a multiple of 32 (checked on the host).
+__global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, const DesignDev* dd) {
+    __shared__ __bf16 sA[2][64][kLdk];
+    __shared__ __bf16 sB[2][64][kLdk];
+    const int j = find_job(jobs, nj, blockIdx.x);
+    const GemmJob& J = jobs[j];
+    const DesignDev& D = dd[J.design];
+    if (D.st->done) return;
+    const int t = blockIdx.x - J.tile_start;
+    const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    f32x16 acc;
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] = 0.0f;
+    const int r = lane & 31, kh = (lane >> 5) * 8;
+    for (int k0 = 0; k0 < J.K; k0 += kTK) {
+        stage_tile(sA, J.a_kind, J.a_kind ? J.A_hi : J.A, J.A_lo, m0, k0, J.K);
+        stage_tile(sB, J.b_kind, J.b_kind ? J.B_hi : J.B, J.B_lo, n0, k0, J.K);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kTK; kk += 16) {
+            const bf16x8 ah = *(const bf16x8*)&sA[0][wm * 32 + r][kk + kh];
+            const bf16x8 al = *(const bf16x8*)&sA[1][wm * 32 + r][kk + kh];
+            const bf16x8 bh = *(const bf16x8*)&sB[0][wn * 32 + r][kk + kh];
+            const bf16x8 bl = *(const bf16x8*)&sB[1][wn * 32 + r][kk + kh];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // accumulator layout: element q of lane l is (row 8*(q/4) + 4*(l/32) + q%4, col l%32)
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int row = m0 + wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+        const int col = n0 + wn * 32 + (lane & 31);
+        epi_store<float>(J, D, row, col, (double)acc[q]);
+    }
+}
+
+void launch_gemm_pass(const GemmJob* jobs, int n_jobs, int total_tiles, const DesignDev* dd,
+                      int prec, int engine, hipStream_t s) {
+    if (n_jobs <= 0 || total_tiles <= 0) return;
+    if (prec == EP_PREC_FP64)
+        hipLaunchKernelGGL(k_gemm_fma<double>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+    else if (engine == 0)
+        hipLaunchKernelGGL(k_gemm_mfma, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+    else
+        hipLaunchKernelGGL(k_gemm_fma<float>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+}
+
+}  // namespace ep

@@ -1,0 +1,152 @@
+// Internal layout shared by the HIP kernels and the host runtime.
+//
+// One resident design owns one device arena.  Every per-iteration kernel
+// covers a whole batch of designs: a block finds its design by binary search
+// over a prefix of per-design block counts, and skips the design when its
+// state says it is done.  All control scalars are FP64 and live on the
+// device, so a batch advances without host round trips.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eplace.h"
+
+namespace ep {
+
+constexpr int kBlock = 256;          // threads per block for cell/bin kernels
+constexpr int kWave = 64;            // CDNA wavefront
+constexpr int kMaxBuckets = 6;       // group sizes 2,4,8,16,32,64
+constexpr uint32_t kPad = 0xFFFFFFFFu;  // empty pin slot
+constexpr int kCellParts = 8;        // partial sums per cell block
+constexpr int kBinParts = 2;         // partial sums per bin block
+constexpr double kFixScale = 4294967296.0;  // 2^32 fixed point for deterministic sums
+
+// Field modes (the transform constants differ, nothing else).
+enum FieldMode : int32_t { kFieldReference = 0, kFieldIdxst = 1 };
+
+// Device-resident control state of one design (one per design).
+struct DevState {
+    double lam;           // density penalty of the current iteration
+    double s[2];          // step sizes (x, y)
+    double a;             // FISTA a_k
+    double m;             // momentum of this iteration's step
+    double m_next;        // momentum of the next look-ahead
+    double ov;            // overflow of this iteration
+    double wl_norm, den_norm;
+    double hpwl;          // last continuous HPWL
+    double final_hpwl;
+    int32_t it;           // iteration being executed (advanced by ctrl_b)
+    int32_t lam_init;     // penalty initialised
+    int32_t need_fix;     // this iteration's penalty was set by ctrl_a
+    int32_t done;         // skip all work
+    int32_t run_update;   // the update of this iteration runs
+    int32_t check;        // this iteration is a convergence-check iteration
+    int32_t converged;
+    int32_t nonfinite;
+    int32_t prev_pending; // step-norm / hpwl partials of the last update await reduction
+    int32_t prev_check;
+    int32_t n_log;
+    int32_t pad0;
+};
+
+// Constant per-design configuration (copied into the descriptor).
+struct DevCfg {
+    double wl_coeff, overflow_target, eta, grow_mult, grow_thr, grow_add, step0;
+    int32_t max_iters, min_iters, interval, stop_every_iter, log_hpwl, check_nonfinite;
+};
+
+// Descriptor of one resident design; pointer fields are typed by precision
+// (T = double for EP_PREC_FP64, float for EP_PREC_FAST).
+struct DesignDev {
+    int32_t n_mov, n_fixed, n_nets, W, H, Bx, By, prec;
+    int32_t n_cell_blocks, n_bin_blocks, n_wa_waves, n_hf, n_buckets, pad1;
+    int32_t bucket_wave_start[kMaxBuckets + 1];
+    int32_t bucket_group[kMaxBuckets];
+    double rx, ry, max_x, max_y;
+    DevCfg cfg;
+    DevState* st;
+    ep_iter_rec* log;          // [max_iters]
+    // cells
+    void* x0[2];               // initial positions (reset source)
+    void* pos[2];              // x, y (solver state)
+    void* dpos[2];             // x - x_prev
+    void* look[2];             // clamped look-ahead
+    void* g[2][2];             // [parity][axis] combined gradient (or density grad)
+    void* gwl[2];              // FP64: double WA gradient accumulators
+    long long* acc[2];         // FAST: fixed-point WA gradient accumulators
+    void* pinw;                // T[n_mov]
+    float* hs;                 // half side of the footprint [n_mov]
+    void* fixed[2];            // T[n_fixed]
+    // nets
+    uint32_t* slots;           // padded pin slots, n_wa_waves * 64
+    uint32_t* hf_off;          // [n_hf + 1]  (nets with > 64 pins)
+    uint32_t* hf_pins;
+    // bins
+    void* dacc;                // FAST: u64 fixed point; FP64: double [By*Bx]
+    void* dens;                // T [By*Bx]
+    void* tT;                  // F1 output, transposed [Bx*By]
+    void* ehat[2];             // F2 output [By*Bx]
+    void* uT[2];               // I1 output, transposed [Bx*By]
+    void* field[2];            // I2 output (Ex, Ey) [By*Bx]
+    void* spec_x;              // T[2*Bx]: (w, 2(1-cos w)) per kx
+    void* spec_y;              // T[2*By]
+    // reductions
+    double* part_cell;         // [n_cell_blocks * kCellParts]
+    double* part_bin;          // [n_bin_blocks * kBinParts]
+    double* part_net;          // [n_wa_waves + n_hf]
+};
+
+// One GEMM of a transform pass: C = A (M x K, row-major) * Bt^T (Bt is
+// N x K row-major).  Operand kinds: 0 = data in T (split to BF16 hi/lo on
+// the fly by the MFMA engine), 1 = constant (T copy plus BF16 hi/lo planes).
+struct GemmJob {
+    const void* A;             // T-typed operand (data, or the constant in T)
+    const void* A_hi;          // FAST constants: bf16 hi and lo planes
+    const void* A_lo;
+    const void* B;
+    const void* B_hi;
+    const void* B_lo;
+    void* C;
+    void* C2;
+    int32_t M, N, K;
+    int32_t a_kind, b_kind;
+    int32_t epi;               // 0: store, 1: store transposed, 2: Poisson epilogue
+    int32_t design;            // index into the descriptor array
+    int32_t tiles_n;           // tiles along N
+    int32_t tile_start;        // first global tile of this job
+    int32_t pad;
+};
+
+enum Epilogue : int32_t { kEpiStore = 0, kEpiStoreT = 1, kEpiPoisson = 2 };
+
+// A batch as the kernels see it (device pointers).
+struct BatchDev {
+    const DesignDev* d;
+    int32_t nd;
+    const int32_t* pfx_cell;   // [nd+1] block prefix for cell kernels
+    const int32_t* pfx_bin;
+    const int32_t* pfx_wa;     // blocks of 4 waves
+    const int32_t* pfx_hf;     // one block per high-degree net
+};
+
+// ---- launchers (ep_kernels.hip) ----
+void launch_wa(const BatchDev& b, int prec, int total_wa_blocks, int total_hf_blocks,
+               hipStream_t s);
+void launch_scatter(const BatchDev& b, int prec, int total_cell_blocks, hipStream_t s);
+void launch_dens_final(const BatchDev& b, int prec, int total_bin_blocks, hipStream_t s);
+void launch_gather(const BatchDev& b, int prec, int total_cell_blocks, int mode, hipStream_t s);
+void launch_ctrl(const BatchDev& b, int which, hipStream_t s);
+void launch_update(const BatchDev& b, int prec, int total_cell_blocks, hipStream_t s);
+void launch_hpwl(const BatchDev& b, int prec, int total_wa_blocks, int total_hf_blocks,
+                 int final_pass, hipStream_t s);
+void launch_reset(const BatchDev& b, int prec, int total_cell_blocks, hipStream_t s);
+void launch_bin_ranges(const BatchDev& b, int total_cell_blocks, int32_t* out, hipStream_t s);
+void launch_gather_export(const BatchDev& b, int prec, int total_cell_blocks, double* gx,
+                          double* gy, hipStream_t s);
+
+// ---- transforms (ep_gemm.hip) ----
+// Runs one pass (all jobs) of the spectral solve; engine 0 = MFMA, 1 = FMA.
+void launch_gemm_pass(const GemmJob* jobs_dev, int n_jobs, int total_tiles,
+                      const DesignDev* dd, int prec, int engine, hipStream_t s);
+
+}  // namespace ep

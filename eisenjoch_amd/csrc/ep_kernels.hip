@@ -1,0 +1,749 @@
+// Per-iteration kernels of the electrostatic placer on CDNA4 (gfx950).
+//
+// Reference loop: crates/nextpnr/src/placer/electro_place/mod.rs:104-197.
+// Each kernel covers a batch of designs; blocks find their design through a
+// prefix of per-design block counts and skip designs whose state is done.
+//
+//   k_wa          WA wirelength gradient (wa.rs:51-101, common.rs:205-265):
+//                 one group of G lanes (G = 2..64, a power of two) per net,
+//                 pins read coalesced from padded slot arrays, max-shifted
+//                 exponentials, group reductions by cross-lane shuffles.
+//   k_wa_hf       the same for nets of more than 64 pins, one block per net.
+//   k_scatter     cell -> bin overlap density (density.rs:54-74); integer bin
+//                 ranges computed in FP64 exactly as the oracle does.
+//   k_dens_final  density accumulator -> density, overflow partials
+//                 (density.rs:79-86), accumulator cleared for the next use.
+//   k_gather      field gather (density.rs:171-206) fused with the penalty
+//                 combine and preconditioner (mod.rs:145-156) and the
+//                 Barzilai-Borwein partial sums (nesterov.rs:199-218).
+//   k_ctrl        FP64 control: overflow, penalty init/growth (mod.rs:131-143),
+//                 BB step (mod.rs:158-166), FISTA momentum (nesterov.rs:41-82),
+//                 convergence (mod.rs:176-195), iteration log.
+//   k_update      Nesterov step + clamp + next look-ahead (nesterov.rs:41-72,
+//                 mod.rs:105-107,169-173).
+//   k_hpwl        continuous HPWL (wirelength.rs:10-51 on continuous positions).
+//
+// Fixed-point (2^32) integer atomics make the FAST path's sums independent of
+// the order in which lanes arrive, so a design's trajectory is the same alone
+// and inside any batch.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "ep_internal.h"
+
+namespace ep {
+
+template <typename T> struct Lim;
+template <> struct Lim<float> { static __device__ float big() { return FLT_MAX; } };
+template <> struct Lim<double> { static __device__ double big() { return DBL_MAX; } };
+
+__device__ inline float fexp(float x) { return __expf(x); }
+__device__ inline double fexp(double x) { return exp(x); }
+
+__device__ inline int find_blk(const int32_t* pfx, int nd, int blk) {
+    int lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (pfx[mid] <= blk) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+__device__ inline T shfl_xor(T v, int o) { return __shfl_xor(v, o, kWave); }
+
+// Deterministic block sum of NV doubles (256 threads); result in lane 0 of wave 0.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double* sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double x = v[q];
+        for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, kWave);
+        v[q] = x;
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) sh[w * NV + q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) v[q] = sh[q] + sh[NV + q] + sh[2 * NV + q] + sh[3 * NV + q];
+    }
+}
+
+__device__ inline double sched(const DevCfg& c, double lam) {
+    return lam < c.grow_thr ? lam * c.grow_mult : lam + c.grow_add;  // mod.rs:139-143
+}
+
+// ---------------------------------------------------------------- WA ----
+
+// Gradient of one pin of a net given the net's shifted sums (exact in R to
+// the reference's mean-centred formula, wa.rs:88-99):
+//   d+ = e/S (1 + a (u - U/S)),  u = x - x_max, e = exp(a u)
+//   d- = f/Q (1 - a (v - V/Q)),  v = x - x_min, f = exp(-a v)
+template <typename T>
+__device__ inline T wa_pin(T a, T u, T e, T S, T U, T v, T f, T Q, T V) {
+    T iS = T(1) / S, iQ = T(1) / Q;
+    return e * iS * (T(1) + a * (u - U * iS)) - f * iQ * (T(1) - a * (v - V * iQ));
+}
+
+template <typename T>
+__device__ inline void pin_pos(const DesignDev& D, uint32_t c, const T* lx, const T* ly, T& px,
+                               T& py) {
+    if (c < (uint32_t)D.n_mov) {
+        px = lx[c];
+        py = ly[c];
+    } else {
+        px = ((const T*)D.fixed[0])[c - D.n_mov];
+        py = ((const T*)D.fixed[1])[c - D.n_mov];
+    }
+}
+
+template <typename T, bool FIX>
+__device__ inline void acc_grad(const DesignDev& D, uint32_t c, T gx, T gy) {
+    if constexpr (FIX) {
+        atomicAdd((unsigned long long*)&D.acc[0][c],
+                  (unsigned long long)__double2ll_rn((double)gx * kFixScale));
+        atomicAdd((unsigned long long*)&D.acc[1][c],
+                  (unsigned long long)__double2ll_rn((double)gy * kFixScale));
+    } else {
+        atomicAdd(&((double*)D.gwl[0])[c], (double)gx);
+        atomicAdd(&((double*)D.gwl[1])[c], (double)gy);
+    }
+}
+
+// HPWL mode: per-wave partial of sum over nets of (max - min) x + y of the
+// solver positions; else the WA gradient of the look-ahead positions.
+template <typename T, bool FIX, bool HPWL>
+__global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
+    const int d = find_blk(b.pfx_wa, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    const DevState* st = D.st;
+    if (HPWL) {
+        if (!final_pass && !(st->run_update && st->check && D.cfg.log_hpwl)) return;
+    } else if (st->done) {
+        return;
+    }
+    const int w = (blockIdx.x - b.pfx_wa[d]) * 4 + (threadIdx.x >> 6);
+    if (w >= D.n_wa_waves) return;  // uniform per wave; no block barrier below
+    const int lane = threadIdx.x & 63;
+    int bk = 0;
+    while (bk + 1 < D.n_buckets && w >= D.bucket_wave_start[bk + 1]) bk++;
+    const int G = D.bucket_group[bk];
+    const uint32_t c = D.slots[(size_t)w * kWave + lane];
+    const bool valid = c != kPad;
+    const T* lx = (const T*)(HPWL ? D.pos[0] : D.look[0]);
+    const T* ly = (const T*)(HPWL ? D.pos[1] : D.look[1]);
+    T px = 0, py = 0;
+    if (valid) pin_pos<T>(D, c, lx, ly, px, py);
+    const T big = Lim<T>::big();
+    T xmx = valid ? px : -big, xmn = valid ? px : big;
+    T ymx = valid ? py : -big, ymn = valid ? py : big;
+    for (int o = 1; o < G; o <<= 1) {
+        xmx = fmax(xmx, shfl_xor(xmx, o));
+        xmn = fmin(xmn, shfl_xor(xmn, o));
+        ymx = fmax(ymx, shfl_xor(ymx, o));
+        ymn = fmin(ymn, shfl_xor(ymn, o));
+    }
+    if (HPWL) {
+        double h = ((lane & (G - 1)) == 0) ? (double)(xmx - xmn) + (double)(ymx - ymn) : 0.0;
+        for (int o = 32; o >= 1; o >>= 1) h += __shfl_xor(h, o, kWave);
+        if (lane == 0) D.part_net[w] = h;
+        return;
+    }
+    const T a = (T)D.cfg.wl_coeff;
+    const T ux = px - xmx, vx = px - xmn, uy = py - ymx, vy = py - ymn;
+    const T ex = valid ? fexp(a * ux) : T(0), fx = valid ? fexp(-a * vx) : T(0);
+    const T ey = valid ? fexp(a * uy) : T(0), fy = valid ? fexp(-a * vy) : T(0);
+    T s[8] = {ex, ux * ex, fx, vx * fx, ey, uy * ey, fy, vy * fy};
+    for (int o = 1; o < G; o <<= 1)
+#pragma unroll
+        for (int q = 0; q < 8; q++) s[q] += shfl_xor(s[q], o);
+    if (valid && c < (uint32_t)D.n_mov) {
+        T gx = wa_pin<T>(a, ux, ex, s[0], s[1], vx, fx, s[2], s[3]);
+        T gy = wa_pin<T>(a, uy, ey, s[4], s[5], vy, fy, s[6], s[7]);
+        acc_grad<T, FIX>(D, c, gx, gy);
+    }
+}
+
+// Nets of more than 64 pins: one block per net, three passes over its pins.
+template <typename T, bool FIX, bool HPWL>
+__global__ __launch_bounds__(256) void k_wa_hf(BatchDev b, int final_pass) {
+    __shared__ double sh[4 * 8];
+    const int d = find_blk(b.pfx_hf, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    const DevState* st = D.st;
+    if (HPWL) {
+        if (!final_pass && !(st->run_update && st->check && D.cfg.log_hpwl)) return;
+    } else if (st->done) {
+        return;
+    }
+    const int e = blockIdx.x - b.pfx_hf[d];
+    if (e >= D.n_hf) return;
+    const uint32_t p0 = D.hf_off[e], p1 = D.hf_off[e + 1];
+    const T* lx = (const T*)(HPWL ? D.pos[0] : D.look[0]);
+    const T* ly = (const T*)(HPWL ? D.pos[1] : D.look[1]);
+    const T big = Lim<T>::big();
+    double mm[4] = {-(double)big, -(double)big, -(double)big, -(double)big};  // xmax, -xmin, ymax, -ymin
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        T px, py;
+        pin_pos<T>(D, D.hf_pins[p], lx, ly, px, py);
+        mm[0] = fmax(mm[0], (double)px);
+        mm[1] = fmax(mm[1], -(double)px);
+        mm[2] = fmax(mm[2], (double)py);
+        mm[3] = fmax(mm[3], -(double)py);
+    }
+    // block max
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            double x = mm[q];
+            for (int o = 32; o >= 1; o >>= 1) x = fmax(x, __shfl_xor(x, o, kWave));
+            if (lane == 0) sh[w * 4 + q] = x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            mm[q] = fmax(fmax(sh[q], sh[4 + q]), fmax(sh[8 + q], sh[12 + q]));
+        __syncthreads();
+    }
+    const T xmx = (T)mm[0], xmn = (T)(-mm[1]), ymx = (T)mm[2], ymn = (T)(-mm[3]);
+    if (HPWL) {
+        if (threadIdx.x == 0)
+            D.part_net[D.n_wa_waves + e] = (double)(xmx - xmn) + (double)(ymx - ymn);
+        return;
+    }
+    const T a = (T)D.cfg.wl_coeff;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        T px, py;
+        pin_pos<T>(D, D.hf_pins[p], lx, ly, px, py);
+        T ux = px - xmx, vx = px - xmn, uy = py - ymx, vy = py - ymn;
+        T ex = fexp(a * ux), fx = fexp(-a * vx), ey = fexp(a * uy), fy = fexp(-a * vy);
+        s[0] += ex; s[1] += ux * ex; s[2] += fx; s[3] += vx * fx;
+        s[4] += ey; s[5] += uy * ey; s[6] += fy; s[7] += vy * fy;
+    }
+    block_sum<8>(s, sh);
+    __syncthreads();
+    __shared__ double tot[8];
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 8; q++) tot[q] = s[q];
+    __syncthreads();
+    T S[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) S[q] = (T)tot[q];
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        uint32_t c = D.hf_pins[p];
+        if (c >= (uint32_t)D.n_mov) continue;
+        T px, py;
+        pin_pos<T>(D, c, lx, ly, px, py);
+        T ux = px - xmx, vx = px - xmn, uy = py - ymx, vy = py - ymn;
+        T ex = fexp(a * ux), fx = fexp(-a * vx), ey = fexp(a * uy), fy = fexp(-a * vy);
+        T gx = wa_pin<T>(a, ux, ex, S[0], S[1], vx, fx, S[2], S[3]);
+        T gy = wa_pin<T>(a, uy, ey, S[4], S[5], vy, fy, S[6], S[7]);
+        acc_grad<T, FIX>(D, c, gx, gy);
+    }
+}
+
+// ----------------------------------------------------------- density ----
+
+struct BinBox {
+    int bx0, bx1, by0, by1;
+    double xl, xh, yl, yh;
+};
+
+// density.rs:20-37 with the area / bin-size extension; FP64 so the integer
+// ranges equal the oracle's bit for bit for the same input positions.
+__device__ inline BinBox bin_box(const DesignDev& D, double cx, double cy, double hs) {
+    BinBox r;
+    r.xl = (cx - hs) * D.rx;
+    r.xh = (cx + hs) * D.rx;
+    r.yl = (cy - hs) * D.ry;
+    r.yh = (cy + hs) * D.ry;
+    int b = (int)floor(r.xl);
+    r.bx0 = b < 0 ? 0 : b;
+    double c = ceil(r.xh);
+    r.bx1 = c <= 0.0 ? 0 : (c >= (double)D.Bx ? D.Bx : (int)c);
+    b = (int)floor(r.yl);
+    r.by0 = b < 0 ? 0 : b;
+    c = ceil(r.yh);
+    r.by1 = c <= 0.0 ? 0 : (c >= (double)D.By ? D.By : (int)c);
+    return r;
+}
+
+// density.rs:41-48
+__device__ inline double ov_area(const BinBox& r, int bx, int by) {
+    double ox = fmin(r.xh, (double)bx + 1.0) - fmax(r.xl, (double)bx);
+    double oy = fmin(r.yh, (double)by + 1.0) - fmax(r.yl, (double)by);
+    ox = ox < 0.0 ? 0.0 : ox;
+    oy = oy < 0.0 ? 0.0 : oy;
+    return ox * oy;
+}
+
+template <typename T, bool FIX>
+__global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
+    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    if (D.st->done) return;
+    const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    if (i >= D.n_mov) return;
+    const double cx = (double)((const T*)D.look[0])[i], cy = (double)((const T*)D.look[1])[i];
+    const BinBox r = bin_box(D, cx, cy, (double)D.hs[i]);
+    for (int by = r.by0; by < r.by1; by++)
+        for (int bx = r.bx0; bx < r.bx1; bx++) {
+            double w = ov_area(r, bx, by);
+            size_t idx = (size_t)by * D.Bx + bx;
+            if constexpr (FIX)
+                atomicAdd(&((unsigned long long*)D.dacc)[idx],
+                          (unsigned long long)__double2ll_rn(w * kFixScale));
+            else
+                atomicAdd(&((double*)D.dacc)[idx], w);
+        }
+}
+
+template <typename T, bool FIX>
+__global__ __launch_bounds__(256) void k_dens_final(BatchDev b) {
+    __shared__ double sh[4 * kBinParts];
+    const int d = find_blk(b.pfx_bin, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    if (D.st->done) return;
+    const int blk = blockIdx.x - b.pfx_bin[d];
+    const int i = blk * kBlock + threadIdx.x;
+    const int nb = D.Bx * D.By;
+    double v = 0.0;
+    if (i < nb) {
+        if constexpr (FIX) {
+            unsigned long long* a = (unsigned long long*)D.dacc;
+            v = (double)(long long)a[i] * (1.0 / kFixScale);
+            a[i] = 0ull;
+        } else {
+            double* a = (double*)D.dacc;
+            v = a[i];
+            a[i] = 0.0;
+        }
+        ((T*)D.dens)[i] = (T)v;
+    }
+    double p[kBinParts] = {v, v > 1.0 ? v - 1.0 : 0.0};
+    block_sum<kBinParts>(p, sh);
+    if (threadIdx.x == 0) {
+        D.part_bin[(size_t)blk * kBinParts + 0] = p[0];
+        D.part_bin[(size_t)blk * kBinParts + 1] = p[1];
+    }
+}
+
+// ------------------------------------------------------------ gather ----
+
+template <typename T, bool FIX>
+__device__ inline double gwl_of(const DesignDev& D, int axis, int i) {
+    if constexpr (FIX) return (double)D.acc[axis][i] * (1.0 / kFixScale);
+    else return ((const double*)D.gwl[axis])[i];
+}
+
+// MODE 0: gather; MODE 1: fix (penalty set by ctrl this iteration);
+// MODE 2: evaluation export of the density gradient only.
+template <typename T, bool FIX, int MODE>
+__global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, double* ey_out) {
+    __shared__ double sh[4 * 4];
+    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    const DevState* st = D.st;
+    if (st->done) return;
+    if (MODE == 1 && !st->need_fix) return;
+    const int blk = blockIdx.x - b.pfx_cell[d];
+    const int i = blk * kBlock + threadIdx.x;
+    const int it = st->it, par = it & 1;
+    double part[4] = {0, 0, 0, 0};
+    if (i < D.n_mov) {
+        double gden[2] = {0, 0};
+        if (MODE != 1) {
+            const double cx = (double)((const T*)D.look[0])[i];
+            const double cy = (double)((const T*)D.look[1])[i];
+            const BinBox r = bin_box(D, cx, cy, (double)D.hs[i]);
+            const T* E0 = (const T*)D.field[0];
+            const T* E1 = (const T*)D.field[1];
+            double fx = 0, fy = 0, wt = 0;
+            for (int by = r.by0; by < r.by1; by++)
+                for (int bx = r.bx0; bx < r.bx1; bx++) {
+                    double w = ov_area(r, bx, by);
+                    if (w > 0.0) {
+                        size_t idx = (size_t)by * D.Bx + bx;
+                        fx += w * (double)E0[idx];
+                        fy += w * (double)E1[idx];
+                        wt += w;
+                    }
+                }
+            if (wt > 1e-10) {
+                gden[0] = fx / wt;
+                gden[1] = fy / wt;
+            }
+        } else {
+            gden[0] = (double)((const T*)D.g[par][0])[i];
+            gden[1] = (double)((const T*)D.g[par][1])[i];
+        }
+        if (MODE == 2) {
+            ex_out[i] = gden[0];
+            ey_out[i] = gden[1];
+        } else if (MODE == 0 && !st->lam_init) {
+            // penalty not yet initialised: park the density gradient, norms only
+            double w0 = gwl_of<T, FIX>(D, 0, i), w1 = gwl_of<T, FIX>(D, 1, i);
+            ((T*)D.g[par][0])[i] = (T)gden[0];
+            ((T*)D.g[par][1])[i] = (T)gden[1];
+            part[0] = w0 * w0 + w1 * w1;
+            part[1] = gden[0] * gden[0] + gden[1] * gden[1];
+        } else {
+            const double lam = MODE == 0 ? sched(D.cfg, st->lam) : st->lam;
+            double pc = ((const T*)D.pinw)[i] + lam;  // mod.rs:152-156
+            pc = pc < 1.0 ? 1.0 : pc;
+#pragma unroll
+            for (int ax = 0; ax < 2; ax++) {
+                double gw = gwl_of<T, FIX>(D, ax, i);
+                T g = (T)((gw + lam * gden[ax]) / pc);  // mod.rs:146-149
+                ((T*)D.g[par][ax])[i] = g;
+                if (it > 0) {
+                    double dx = (double)((const T*)D.dpos[ax])[i];
+                    double dg = (double)g - (double)((const T*)D.g[par ^ 1][ax])[i];
+                    part[2 * ax + 0] = dx * dx;
+                    part[2 * ax + 1] = dx * dg;
+                }
+            }
+        }
+    }
+    if (MODE == 2) return;
+    block_sum<4>(part, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 4; q++) D.part_cell[(size_t)blk * kCellParts + q] = part[q];
+}
+
+// ------------------------------------------------------------ control ---
+
+__device__ inline void bsum_range(const double* p, int n, int stride, int off, double& out,
+                                  double* sh) {
+    // fixed-order strided sum then a fixed tree: deterministic
+    double v[1] = {0.0};
+    for (int k = threadIdx.x; k < n; k += blockDim.x) v[0] += p[(size_t)k * stride + off];
+    block_sum<1>(v, sh);
+    __syncthreads();
+    if (threadIdx.x == 0) sh[16] = v[0];
+    __syncthreads();
+    out = sh[16];
+    __syncthreads();
+}
+
+// Reduce the partials of the previous update (step norms, HPWL) into its log record.
+__device__ inline void reduce_prev(const DesignDev& D, DevState* st, double* sh) {
+    if (!st->prev_pending) return;
+    double sx, sy;
+    bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 4, sx, sh);
+    bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 5, sy, sh);
+    double h = 0.0;
+    bool hp = st->prev_check && D.cfg.log_hpwl;
+    if (hp) bsum_range(D.part_net, D.n_wa_waves + D.n_hf, 1, 0, h, sh);
+    if (threadIdx.x == 0) {
+        const int k = st->it - 1;
+        if (k >= 0 && k < D.cfg.max_iters) {
+            D.log[k].step_norm_x = sqrt(sx);
+            D.log[k].step_norm_y = sqrt(sy);
+            D.log[k].hpwl = hp ? h : 0.0;
+        }
+        if (hp) st->hpwl = h;
+        st->prev_pending = 0;
+    }
+    __syncthreads();
+}
+
+// which = 0: after the gather; 1: after the fix (end of control);
+// 2: finalize pending log partials; 3: reduce a final HPWL pass.
+__global__ __launch_bounds__(256) void k_ctrl(BatchDev b, int which) {
+    __shared__ double sh[32];
+    const DesignDev& D = b.d[blockIdx.x];
+    DevState* st = D.st;
+    const DevCfg& c = D.cfg;
+    if (which == 2) {
+        reduce_prev(D, st, sh);
+        return;
+    }
+    if (which == 3) {
+        double h;
+        bsum_range(D.part_net, D.n_wa_waves + D.n_hf, 1, 0, h, sh);
+        if (threadIdx.x == 0) st->final_hpwl = h;
+        return;
+    }
+    if (st->done) {
+        if (which == 1 && threadIdx.x == 0) st->run_update = 0;
+        return;
+    }
+    const int k = st->it;
+    if (which == 0) {
+        reduce_prev(D, st, sh);
+        double tot, ovs, p0, p1, p2, p3;
+        bsum_range(D.part_bin, D.n_bin_blocks, kBinParts, 0, tot, sh);
+        bsum_range(D.part_bin, D.n_bin_blocks, kBinParts, 1, ovs, sh);
+        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 0, p0, sh);
+        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 1, p1, sh);
+        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 2, p2, sh);
+        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 3, p3, sh);
+        if (threadIdx.x != 0) return;
+        const double ov = tot < 1e-30 ? 0.0 : ovs / tot;  // density.rs:79-86
+        st->ov = ov;
+        double wln = 0.0, dnn = 0.0;
+        if (st->lam_init) {
+            st->lam = sched(c, st->lam);
+            if (k > 0) {  // mod.rs:159-166 with nesterov.rs:199-218
+                if (fabs(p1) >= 1e-30) st->s[0] = fmin(fmax(fabs(p0 / p1), 1e-4), 1.0);
+                if (fabs(p3) >= 1e-30) st->s[1] = fmin(fmax(fabs(p2 / p3), 1e-4), 1.0);
+            }
+            st->need_fix = 0;
+        } else {
+            wln = sqrt(p0);  // common.rs:418-425
+            dnn = sqrt(p1);
+            if (dnn > 1e-30) {  // mod.rs:132-138
+                st->lam = c.eta * wln / dnn;
+                st->lam_init = 1;
+            }
+            st->need_fix = 1;
+        }
+        st->wl_norm = wln;
+        st->den_norm = dnn;
+        st->check = c.stop_every_iter || (k % c.interval == 0) || (k == c.max_iters - 1);
+        if (c.check_nonfinite && !(isfinite(ov) && isfinite(st->lam))) st->nonfinite = 1;
+        ep_iter_rec& r = D.log[k];
+        r.iter = k;
+        r.checked = st->check;
+        r.overflow = ov;
+        r.lambda = st->lam;
+        r.wl_norm = wln;
+        r.den_norm = dnn;
+        r.step_norm_x = r.step_norm_y = 0.0;
+        r.hpwl = 0.0;
+        return;
+    }
+    // which == 1
+    if (st->need_fix) {
+        double p0, p1, p2, p3;
+        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 0, p0, sh);
+        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 1, p1, sh);
+        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 2, p2, sh);
+        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 3, p3, sh);
+        if (threadIdx.x == 0 && k > 0) {
+            if (fabs(p1) >= 1e-30) st->s[0] = fmin(fmax(fabs(p0 / p1), 1e-4), 1.0);
+            if (fabs(p3) >= 1e-30) st->s[1] = fmin(fmax(fabs(p2 / p3), 1e-4), 1.0);
+        }
+    }
+    if (threadIdx.x != 0) return;
+    st->need_fix = 0;
+    // FISTA sequence (nesterov.rs:41-47 and :75-82): the step of iteration k
+    // uses m_k = (a_k - 1) / a_{k+1}; the next look-ahead uses m_{k+1}.
+    const double a = st->a;
+    const double an = (1.0 + sqrt(1.0 + 4.0 * a * a)) / 2.0;
+    const double ann = (1.0 + sqrt(1.0 + 4.0 * an * an)) / 2.0;
+    st->m = (a - 1.0) / an;
+    st->m_next = (an - 1.0) / ann;
+    st->a = an;
+    ep_iter_rec& r = D.log[k];
+    r.step_size_x = st->s[0];
+    r.step_size_y = st->s[1];
+    bool stop = false;
+    if (st->check && k >= c.min_iters && st->ov < c.overflow_target) {  // mod.rs:192-195
+        st->converged = 1;
+        stop = true;
+    }
+    if (k + 1 >= c.max_iters) stop = true;
+    if (st->nonfinite) stop = true;
+    st->run_update = 1;
+    st->prev_pending = 1;
+    st->prev_check = st->check;
+    st->it = k + 1;
+    st->n_log = k + 1;
+    if (stop) st->done = 1;
+}
+
+// ------------------------------------------------------------ update ----
+
+template <typename T, bool FIX>
+__global__ __launch_bounds__(256) void k_update(BatchDev b) {
+    __shared__ double sh[4 * 2];
+    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    const DevState* st = D.st;
+    if (!st->run_update) return;
+    const int blk = blockIdx.x - b.pfx_cell[d];
+    const int i = blk * kBlock + threadIdx.x;
+    const int par = (st->it - 1) & 1;
+    double sq[2] = {0, 0};
+    if (i < D.n_mov) {
+        const T m = (T)st->m, mn = (T)st->m_next;
+#pragma unroll
+        for (int ax = 0; ax < 2; ax++) {
+            const T hi = (T)(ax == 0 ? D.max_x : D.max_y);
+            const T s = (T)st->s[ax];
+            T* P = (T*)D.pos[ax];
+            T* DP = (T*)D.dpos[ax];
+            const T x = P[i], dx = DP[i];
+            const T g = ((const T*)D.g[par][ax])[i];
+            const T yv = x + m * dx;  // nesterov.rs:55: y = x + m (x - x_prev)
+            const T xn = yv - s * g;  // nesterov.rs:56
+            const double dl = (double)xn - (double)x;
+            sq[ax] = dl * dl;
+            const T xnc = xn < T(0) ? T(0) : (xn > hi ? hi : xn);  // nesterov.rs:67-72
+            const T xpc = x < T(0) ? T(0) : (x > hi ? hi : x);
+            P[i] = xnc;
+            const T ndx = xnc - xpc;
+            DP[i] = ndx;
+            T l = xnc + mn * ndx;  // nesterov.rs:95-101 then common.rs:410-415
+            l = l < T(0) ? T(0) : (l > hi ? hi : l);
+            ((T*)D.look[ax])[i] = l;
+            if constexpr (FIX) D.acc[ax][i] = 0;
+            else ((double*)D.gwl[ax])[i] = 0.0;
+        }
+    }
+    block_sum<2>(sq, sh);
+    if (threadIdx.x == 0) {
+        D.part_cell[(size_t)blk * kCellParts + 4] = sq[0];
+        D.part_cell[(size_t)blk * kCellParts + 5] = sq[1];
+    }
+}
+
+// Reset a design to its initial positions (NesterovSolver::set_positions,
+// nesterov.rs:114-118; first look-ahead has zero momentum, nesterov.rs:75-82).
+template <typename T, bool FIX>
+__global__ __launch_bounds__(256) void k_reset(BatchDev b) {
+    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    if (i >= D.n_mov) return;
+#pragma unroll
+    for (int ax = 0; ax < 2; ax++) {
+        const T hi = (T)(ax == 0 ? D.max_x : D.max_y);
+        const T x = ((const T*)D.x0[ax])[i];
+        ((T*)D.pos[ax])[i] = x;
+        ((T*)D.dpos[ax])[i] = T(0);
+        ((T*)D.look[ax])[i] = x < T(0) ? T(0) : (x > hi ? hi : x);
+        if constexpr (FIX) D.acc[ax][i] = 0;
+        else ((double*)D.gwl[ax])[i] = 0.0;
+    }
+}
+
+// Evaluation helper: integer bin ranges per cell.
+__global__ __launch_bounds__(256) void k_bin_ranges(BatchDev b, int32_t* out) {
+    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    if (i >= D.n_mov) return;
+    double cx, cy;
+    if (D.prec == EP_PREC_FP64) {
+        cx = ((const double*)D.look[0])[i];
+        cy = ((const double*)D.look[1])[i];
+    } else {
+        cx = (double)((const float*)D.look[0])[i];
+        cy = (double)((const float*)D.look[1])[i];
+    }
+    BinBox r = bin_box(D, cx, cy, (double)D.hs[i]);
+    out[4 * i + 0] = r.bx0;
+    out[4 * i + 1] = r.bx1;
+    out[4 * i + 2] = r.by0;
+    out[4 * i + 3] = r.by1;
+}
+
+// ----------------------------------------------------------- launchers --
+
+void launch_wa(const BatchDev& b, int prec, int wa_blocks, int hf_blocks, hipStream_t s) {
+    if (wa_blocks > 0) {
+        if (prec == EP_PREC_FP64)
+            hipLaunchKernelGGL((k_wa<double, false, false>), dim3(wa_blocks), dim3(256), 0, s, b, 0);
+        else
+            hipLaunchKernelGGL((k_wa<float, true, false>), dim3(wa_blocks), dim3(256), 0, s, b, 0);
+    }
+    if (hf_blocks > 0) {
+        if (prec == EP_PREC_FP64)
+            hipLaunchKernelGGL((k_wa_hf<double, false, false>), dim3(hf_blocks), dim3(256), 0, s, b, 0);
+        else
+            hipLaunchKernelGGL((k_wa_hf<float, true, false>), dim3(hf_blocks), dim3(256), 0, s, b, 0);
+    }
+}
+
+void launch_hpwl(const BatchDev& b, int prec, int wa_blocks, int hf_blocks, int final_pass,
+                 hipStream_t s) {
+    if (wa_blocks > 0) {
+        if (prec == EP_PREC_FP64)
+            hipLaunchKernelGGL((k_wa<double, false, true>), dim3(wa_blocks), dim3(256), 0, s, b, final_pass);
+        else
+            hipLaunchKernelGGL((k_wa<float, true, true>), dim3(wa_blocks), dim3(256), 0, s, b, final_pass);
+    }
+    if (hf_blocks > 0) {
+        if (prec == EP_PREC_FP64)
+            hipLaunchKernelGGL((k_wa_hf<double, false, true>), dim3(hf_blocks), dim3(256), 0, s, b, final_pass);
+        else
+            hipLaunchKernelGGL((k_wa_hf<float, true, true>), dim3(hf_blocks), dim3(256), 0, s, b, final_pass);
+    }
+}
+
+void launch_scatter(const BatchDev& b, int prec, int cell_blocks, hipStream_t s) {
+    if (cell_blocks <= 0) return;
+    if (prec == EP_PREC_FP64)
+        hipLaunchKernelGGL((k_scatter<double, false>), dim3(cell_blocks), dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL((k_scatter<float, true>), dim3(cell_blocks), dim3(256), 0, s, b);
+}
+
+void launch_dens_final(const BatchDev& b, int prec, int bin_blocks, hipStream_t s) {
+    if (bin_blocks <= 0) return;
+    if (prec == EP_PREC_FP64)
+        hipLaunchKernelGGL((k_dens_final<double, false>), dim3(bin_blocks), dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL((k_dens_final<float, true>), dim3(bin_blocks), dim3(256), 0, s, b);
+}
+
+void launch_gather(const BatchDev& b, int prec, int cell_blocks, int mode, hipStream_t s) {
+    if (cell_blocks <= 0) return;
+    double* n = nullptr;
+    if (prec == EP_PREC_FP64) {
+        if (mode == 0) hipLaunchKernelGGL((k_gather<double, false, 0>), dim3(cell_blocks), dim3(256), 0, s, b, n, n);
+        else hipLaunchKernelGGL((k_gather<double, false, 1>), dim3(cell_blocks), dim3(256), 0, s, b, n, n);
+    } else {
+        if (mode == 0) hipLaunchKernelGGL((k_gather<float, true, 0>), dim3(cell_blocks), dim3(256), 0, s, b, n, n);
+        else hipLaunchKernelGGL((k_gather<float, true, 1>), dim3(cell_blocks), dim3(256), 0, s, b, n, n);
+    }
+}
+
+void launch_gather_export(const BatchDev& b, int prec, int cell_blocks, double* gx, double* gy,
+                          hipStream_t s) {
+    if (cell_blocks <= 0) return;
+    if (prec == EP_PREC_FP64)
+        hipLaunchKernelGGL((k_gather<double, false, 2>), dim3(cell_blocks), dim3(256), 0, s, b, gx, gy);
+    else
+        hipLaunchKernelGGL((k_gather<float, true, 2>), dim3(cell_blocks), dim3(256), 0, s, b, gx, gy);
+}
+
+void launch_ctrl(const BatchDev& b, int which, hipStream_t s) {
+    if (b.nd <= 0) return;
+    hipLaunchKernelGGL(k_ctrl, dim3(b.nd), dim3(256), 0, s, b, which);
+}
+
+void launch_update(const BatchDev& b, int prec, int cell_blocks, hipStream_t s) {
+    if (cell_blocks <= 0) return;
+    if (prec == EP_PREC_FP64)
+        hipLaunchKernelGGL((k_update<double, false>), dim3(cell_blocks), dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL((k_update<float, true>), dim3(cell_blocks), dim3(256), 0, s, b);
+}
+
+void launch_reset(const BatchDev& b, int prec, int cell_blocks, hipStream_t s) {
+    if (cell_blocks <= 0) return;
+    if (prec == EP_PREC_FP64)
+        hipLaunchKernelGGL((k_reset<double, false>), dim3(cell_blocks), dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL((k_reset<float, true>), dim3(cell_blocks), dim3(256), 0, s, b);
+}
+
+void launch_bin_ranges(const BatchDev& b, int cell_blocks, int32_t* out, hipStream_t s) {
+    if (cell_blocks <= 0) return;
+    hipLaunchKernelGGL(k_bin_ranges, dim3(cell_blocks), dim3(256), 0, s, b, out);
+}
+
+}  // namespace ep

@@ -1,0 +1,670 @@
+// Batches, the placement loop (mod.rs:104-197) and single-kernel evaluation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ep_internal.h"
+#include "ep_runtime.h"
+
+namespace ep {
+
+#define HIPCHK(ctx, expr)                                                             \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(ctx, EP_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static DevCfg make_cfg(const ep_config* c) {
+    DevCfg d;
+    d.wl_coeff = c->wl_coeff;
+    d.overflow_target = c->overflow_target;
+    d.eta = c->density_eta;
+    d.grow_mult = c->grow_mult;
+    d.grow_thr = c->grow_threshold;
+    d.grow_add = c->grow_add;
+    d.step0 = c->nesterov_step_size;
+    d.max_iters = c->max_iters;
+    d.min_iters = c->min_iters;
+    d.interval = c->legalize_interval > 0 ? c->legalize_interval : 1;
+    d.stop_every_iter = c->stop_every_iter;
+    d.log_hpwl = c->log_hpwl;
+    d.check_nonfinite = c->check_nonfinite;
+    return d;
+}
+
+int validate_config(Ctx* c, const ep_config* cfg, int log_cap) {
+    if (!cfg) return fail(c, EP_ERR_INVALID_ARG, "null config");
+    if (cfg->max_iters < 0 || cfg->max_iters > log_cap)
+        return fail(c, EP_ERR_INVALID_ARG, "max_iters out of range (0.." + std::to_string(log_cap) + ")");
+    if (!(cfg->wl_coeff > 0.0) || !std::isfinite(cfg->wl_coeff))
+        return fail(c, EP_ERR_INVALID_ARG, "wl_coeff must be positive");
+    if (cfg->field_mode != EP_FIELD_REFERENCE && cfg->field_mode != EP_FIELD_IDXST)
+        return fail(c, EP_ERR_INVALID_ARG, "unknown field mode");
+    return EP_OK;
+}
+
+// Per pass: (A kind, A matrix or buffer, B kind, B matrix or buffer, epilogue, out, out2)
+int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<const ep_config*>& cfgs,
+                     Batch& b) {
+    batch_release(b);
+    b.ds = ds;
+    const int nd = (int)ds.size();
+    if (nd == 0) return EP_OK;
+    b.prec = ds[0]->prec;
+    b.log_hpwl = 0;
+    for (int i = 0; i < nd; i++) {
+        if (ds[i]->prec != b.prec) return fail(c, EP_ERR_INVALID_ARG, "mixed precisions in one batch");
+        if (cfgs[i]->log_hpwl) b.log_hpwl = 1;
+    }
+    std::vector<DesignDev> dv(nd);
+    std::vector<int32_t> pc(nd + 1), pb(nd + 1), pw(nd + 1), ph(nd + 1);
+    pc[0] = pb[0] = pw[0] = ph[0] = 0;
+    // jobs: [pass][engine]
+    std::vector<GemmJob> jobs[4][2];
+    for (int i = 0; i < nd; i++) {
+        Design& D = *ds[i];
+        dv[i] = D.dev;
+        dv[i].cfg = make_cfg(cfgs[i]);
+        pc[i + 1] = pc[i] + D.dev.n_cell_blocks;
+        pb[i + 1] = pb[i] + D.dev.n_bin_blocks;
+        pw[i + 1] = pw[i] + (D.dev.n_wa_waves + 3) / 4;
+        ph[i + 1] = ph[i] + D.dev.n_hf;
+        Transform *tx, *ty;
+        int rc = get_transform(c, D.dev.Bx, cfgs[i]->field_mode, &tx);
+        if (!rc) rc = get_transform(c, D.dev.By, cfgs[i]->field_mode, &ty);
+        if (rc) return rc;
+        const bool f64 = b.prec == EP_PREC_FP64;
+        auto cst = [&](GemmJob& J, bool isA, Transform* t, int k) {
+            const void* p = f64 ? (const void*)t->d[k] : (const void*)t->f[k];
+            if (isA) { J.A = p; J.A_hi = t->hi[k]; J.A_lo = t->lo[k]; J.a_kind = 1; }
+            else { J.B = p; J.B_hi = t->hi[k]; J.B_lo = t->lo[k]; J.b_kind = 1; }
+        };
+        auto dat = [&](GemmJob& J, bool isA, const void* p) {
+            if (isA) { J.A = p; J.A_hi = J.A_lo = nullptr; J.a_kind = 0; }
+            else { J.B = p; J.B_hi = J.B_lo = nullptr; J.b_kind = 0; }
+        };
+        const int Bx = D.dev.Bx, By = D.dev.By;
+        auto mk = [&](int M, int N, int K, int epi, void* C, void* C2) {
+            GemmJob J;
+            memset(&J, 0, sizeof(J));
+            J.M = M; J.N = N; J.K = K; J.epi = epi; J.C = C; J.C2 = C2; J.design = i;
+            J.tiles_n = (N + 63) / 64;
+            return J;
+        };
+        GemmJob f1 = mk(By, Bx, Bx, kEpiStoreT, D.dev.tT, nullptr);  // d . C2^T   (rows)
+        dat(f1, true, D.dev.dens); cst(f1, false, tx, 0);
+        GemmJob f2 = mk(By, Bx, By, kEpiPoisson, D.dev.ehat[0], D.dev.ehat[1]);  // C2 . T (cols)
+        cst(f2, true, ty, 0); dat(f2, false, D.dev.tT);
+        GemmJob i1a = mk(By, Bx, Bx, kEpiStoreT, D.dev.uT[0], nullptr);  // Ex^ . S3^T (idst_rows)
+        dat(i1a, true, D.dev.ehat[0]); cst(i1a, false, tx, 2);
+        GemmJob i1b = mk(By, Bx, Bx, kEpiStoreT, D.dev.uT[1], nullptr);  // Ey^ . C3^T (idct_rows)
+        dat(i1b, true, D.dev.ehat[1]); cst(i1b, false, tx, 1);
+        GemmJob i2a = mk(By, Bx, By, kEpiStore, D.dev.field[0], nullptr);  // C3 . U (idct_cols)
+        cst(i2a, true, ty, 1); dat(i2a, false, D.dev.uT[0]);
+        GemmJob i2b = mk(By, Bx, By, kEpiStore, D.dev.field[1], nullptr);  // S3 . U (idst_cols)
+        cst(i2b, true, ty, 2); dat(i2b, false, D.dev.uT[1]);
+        const bool mfma_ok = b.prec == EP_PREC_FAST && Bx % 64 == 0 && By % 64 == 0 &&
+                             getenv("EP_FORCE_FMA") == nullptr;
+        const int eng = mfma_ok ? 0 : 1;
+        jobs[0][eng].push_back(f1);
+        jobs[1][eng].push_back(f2);
+        jobs[2][eng].push_back(i1a);
+        jobs[2][eng].push_back(i1b);
+        jobs[3][eng].push_back(i2a);
+        jobs[3][eng].push_back(i2b);
+    }
+    b.cell_blocks = pc[nd];
+    b.bin_blocks = pb[nd];
+    b.wa_blocks = pw[nd];
+    b.hf_blocks = ph[nd];
+    // device image: descriptors, prefixes, jobs
+    size_t bytes = align_up(sizeof(DesignDev) * nd) + 4 * align_up(sizeof(int32_t) * (nd + 1)) + align_up(4);
+    for (int p = 0; p < 4; p++)
+        for (int e = 0; e < 2; e++) {
+            int t0 = 0;
+            for (auto& J : jobs[p][e]) {
+                J.tile_start = t0;
+                t0 += ((J.M + 63) / 64) * J.tiles_n;
+            }
+            b.tiles2[p][e] = t0;
+            b.njobs2[p][e] = (int)jobs[p][e].size();
+            bytes += align_up(sizeof(GemmJob) * std::max<size_t>(1, jobs[p][e].size()));
+        }
+    HIPCHK(c, hipMalloc(&b.mem, bytes));
+    std::vector<char> img(bytes, 0);
+    size_t off = 0;
+    auto put = [&](const void* src, size_t n) {
+        size_t o = off;
+        memcpy(img.data() + o, src, n);
+        off += align_up(n);
+        return (char*)b.mem + o;
+    };
+    b.d_dev = (DesignDev*)put(dv.data(), sizeof(DesignDev) * nd);
+    b.bd.d = b.d_dev;
+    b.bd.nd = nd;
+    b.bd.pfx_cell = (const int32_t*)put(pc.data(), 4 * (nd + 1));
+    b.bd.pfx_bin = (const int32_t*)put(pb.data(), 4 * (nd + 1));
+    b.bd.pfx_wa = (const int32_t*)put(pw.data(), 4 * (nd + 1));
+    b.bd.pfx_hf = (const int32_t*)put(ph.data(), 4 * (nd + 1));
+    const int32_t zero = 0;
+    b.dzero = (const int32_t*)put(&zero, 4);
+    for (int p = 0; p < 4; p++)
+        for (int e = 0; e < 2; e++) {
+            GemmJob dummy;
+            memset(&dummy, 0, sizeof(dummy));
+            if (jobs[p][e].empty()) b.jobs2[p][e] = (GemmJob*)put(&dummy, sizeof(GemmJob));
+            else b.jobs2[p][e] = (GemmJob*)put(jobs[p][e].data(), sizeof(GemmJob) * jobs[p][e].size());
+        }
+    HIPCHK(c, hipMemcpy(b.mem, img.data(), bytes, hipMemcpyHostToDevice));
+    b.cfgs.assign(nd, ep_config{});
+    for (int i = 0; i < nd; i++) b.cfgs[i] = *cfgs[i];
+    return EP_OK;
+}
+
+int batch_build(Ctx* c, const std::vector<Design*>& ds, const ep_config* cfg, Batch& b) {
+    std::vector<const ep_config*> cf(ds.size(), cfg);
+    return batch_build_cfgs(c, ds, cf, b);
+}
+
+void batch_release(Batch& b) {
+    if (b.graph) hipGraphExecDestroy(b.graph);
+    b.graph = nullptr;
+    b.graph_iters = 0;
+    if (b.mem) hipFree(b.mem);
+    b.mem = nullptr;
+    b.ds.clear();
+}
+
+static DevState fresh_state(const Design& D, const ep_config* cfg) {
+    DevState s;
+    memset(&s, 0, sizeof(s));
+    s.lam = 0.0;  // mod.rs:100-101
+    s.s[0] = s.s[1] = cfg->nesterov_step_size;
+    s.a = 1.0;  // nesterov.rs:30
+    s.done = (D.dev.n_mov == 0 || cfg->max_iters == 0) ? 1 : 0;  // mod.rs:80-82
+    return s;
+}
+
+// Reset the listed designs of a batch (all when idx is empty).
+int batch_reset_some(Ctx* c, Batch& b, const std::vector<int>& idx) {
+    std::vector<int> which = idx;
+    if (which.empty())
+        for (int i = 0; i < (int)b.ds.size(); i++) which.push_back(i);
+    for (int i : which) {
+        Design& D = *b.ds[i];
+        c->state_host[D.state_idx] = fresh_state(D, &b.cfgs[i]);
+        HIPCHK(c, hipMemcpyAsync(D.dev.st, &c->state_host[D.state_idx], sizeof(DevState),
+                                 hipMemcpyHostToDevice, c->stream));
+    }
+    // positions: a reset launch over the whole batch would disturb running
+    // designs, so reset only the listed ones, one small batch each
+    if (which.size() == b.ds.size()) {
+        launch_reset(b.bd, b.prec, b.cell_blocks, c->stream);
+    } else {
+        for (int i : which) {
+            BatchDev one = b.bd;
+            one.d = b.d_dev + i;
+            one.nd = 1;
+            one.pfx_cell = b.dzero;
+            launch_reset(one, b.prec, b.ds[i]->dev.n_cell_blocks, c->stream);
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EP_OK;
+}
+
+int batch_reset(Ctx* c, Batch& b, const ep_config* cfg) {
+    (void)cfg;
+    return batch_reset_some(c, b, {});
+}
+
+static void enqueue_gemm(Ctx* c, Batch& b, int p) {
+    for (int e = 0; e < 2; e++)
+        if (b.njobs2[p][e] > 0)
+            launch_gemm_pass(b.jobs2[p][e], b.njobs2[p][e], b.tiles2[p][e], b.d_dev, b.prec, e, c->stream);
+}
+
+// One iteration of mod.rs:104-197 for every live design of the batch.
+static void enqueue_iteration(Ctx* c, Batch& b) {
+    hipStream_t s = c->stream;
+    launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, s);     // mod.rs:110-115
+    launch_scatter(b.bd, b.prec, b.cell_blocks, s);           // mod.rs:118
+    launch_dens_final(b.bd, b.prec, b.bin_blocks, s);         // mod.rs:119
+    for (int p = 0; p < 4; p++) enqueue_gemm(c, b, p);        // mod.rs:121-123
+    launch_gather(b.bd, b.prec, b.cell_blocks, 0, s);         // mod.rs:126-156
+    launch_ctrl(b.bd, 0, s);                                  // mod.rs:131-143, 159-166
+    launch_gather(b.bd, b.prec, b.cell_blocks, 1, s);
+    launch_ctrl(b.bd, 1, s);                                  // nesterov.rs:41-47, mod.rs:176-195
+    launch_update(b.bd, b.prec, b.cell_blocks, s);            // mod.rs:169-173, 105-107
+    if (b.log_hpwl) launch_hpwl(b.bd, b.prec, b.wa_blocks, b.hf_blocks, 0, s);
+}
+
+int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph) {
+    if (b.ds.empty() || iters <= 0) return EP_OK;
+    if (!use_graph) {
+        for (int i = 0; i < iters; i++) enqueue_iteration(c, b);
+        HIPCHK(c, hipGetLastError());
+        return EP_OK;
+    }
+    if (!b.graph || b.graph_iters != iters) {
+        if (b.graph) hipGraphExecDestroy(b.graph);
+        b.graph = nullptr;
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < iters; i++) enqueue_iteration(c, b);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&b.graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+        b.graph_iters = iters;
+    }
+    HIPCHK(c, hipGraphLaunch(b.graph, c->stream));
+    return EP_OK;
+}
+
+int read_states(Ctx* c, const Batch& b) {
+    if (b.ds.empty()) return EP_OK;
+    int lo = kStatePool, hi = -1;
+    for (auto* D : b.ds) {
+        lo = std::min(lo, D->state_idx);
+        hi = std::max(hi, D->state_idx);
+    }
+    HIPCHK(c, hipMemcpyAsync(&c->state_host[lo], &c->state_pool[lo], sizeof(DevState) * (hi - lo + 1),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EP_OK;
+}
+
+int batch_finalize(Ctx* c, Batch& b) {
+    launch_ctrl(b.bd, 2, c->stream);
+    launch_hpwl(b.bd, b.prec, b.wa_blocks, b.hf_blocks, 1, c->stream);
+    launch_ctrl(b.bd, 3, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return read_states(c, b);
+}
+
+void fill_result(const Design& D, const DevState& s, ep_result* r) {
+    (void)D;
+    r->status = s.nonfinite ? EP_ERR_NONFINITE : EP_OK;
+    r->iters = s.it;
+    r->converged = s.converged;
+    r->precision = D.prec;
+    r->overflow = s.ov;
+    r->hpwl = s.final_hpwl;
+    r->lambda = s.lam;
+}
+
+// Runs a built batch until every design stops.
+int batch_place(Ctx* c, Batch& b, int sync_every, bool use_graph) {
+    int rc = batch_reset(c, b, nullptr);
+    if (rc) return rc;
+    int max_it = 0;
+    for (auto& cf : b.cfgs) max_it = std::max(max_it, cf.max_iters);
+    int done_iters = 0;
+    for (;;) {
+        bool all = true;
+        if ((rc = read_states(c, b))) return rc;
+        for (auto* D : b.ds) all = all && c->state_host[D->state_idx].done;
+        if (all || done_iters >= max_it) break;
+        if ((rc = batch_run_iters(c, b, sync_every, use_graph))) return rc;
+        done_iters += sync_every;
+    }
+    return batch_finalize(c, b);
+}
+
+Design* find_design(Ctx* c, uint32_t id) {
+    auto it = c->designs.find(id);
+    return it == c->designs.end() ? nullptr : it->second.get();
+}
+
+// ------------------------------------------------------------ caching ----
+
+static Batch* cached_batch(Ctx* c, const std::vector<Design*>& ds, const ep_config* cfg, int* rc) {
+    *rc = EP_OK;
+    if (c->cache && c->cache->ds == ds && c->cache_cfg_valid &&
+        memcmp(&c->cache_cfg, cfg, sizeof(ep_config)) == 0)
+        return c->cache.get();
+    release_batch_cache(c);
+    c->cache.reset(new Batch());
+    *rc = batch_build(c, ds, cfg, *c->cache);
+    if (*rc) {
+        release_batch_cache(c);
+        return nullptr;
+    }
+    c->cache_cfg = *cfg;
+    c->cache_cfg_valid = true;
+    return c->cache.get();
+}
+
+void release_batch_cache(Ctx* c) {
+    if (c->cache) batch_release(*c->cache);
+    c->cache.reset();
+    c->cache_cfg_valid = false;
+}
+
+int free_design_locked(Ctx* c, uint32_t id) {
+    auto it = c->designs.find(id);
+    if (it == c->designs.end()) return fail(c, EP_ERR_NOT_FOUND, "unknown design id");
+    if (c->cache)
+        for (auto* D : c->cache->ds)
+            if (D->id == id) { release_batch_cache(c); break; }
+    hipStreamSynchronize(c->stream);
+    hipFree(it->second->arena);
+    c->used -= it->second->bytes;
+    c->free_states.push_back(it->second->state_idx);
+    c->designs.erase(it);
+    return EP_OK;
+}
+
+}  // namespace ep
+
+using namespace ep;
+
+static bool use_graphs() { return getenv("EP_NO_GRAPH") == nullptr; }
+
+extern "C" {
+
+int ep_design_free(ep_ctx* h, uint32_t id) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return fail(nullptr, EP_ERR_INVALID_ARG, "null context");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    return free_design_locked(c, id);
+}
+
+int ep_place_batch(ep_ctx* h, const uint32_t* ids, uint32_t n, const ep_config* cfg, ep_result* res) {
+    Ctx* c = (Ctx*)h;
+    if (!c || (!ids && n) || !cfg || (!res && n)) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    if (c->server_running) return fail(c, EP_ERR_INVALID_ARG, "context is serving; stop it first");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<Design*> ds;
+    for (uint32_t i = 0; i < n; i++) {
+        Design* D = find_design(c, ids[i]);
+        if (!D) return fail(c, EP_ERR_NOT_FOUND, "unknown design id");
+        int rc = validate_config(c, cfg, D->log_cap);
+        if (rc) return rc;
+        ds.push_back(D);
+    }
+    if (ds.empty()) return EP_OK;
+    int rc;
+    Batch* b = cached_batch(c, ds, cfg, &rc);
+    if (!b) return rc;
+    const int sync_every = cfg->legalize_interval > 0 ? cfg->legalize_interval : 5;
+    rc = batch_place(c, *b, sync_every, use_graphs());
+    if (rc) return rc;
+    int worst = EP_OK;
+    for (uint32_t i = 0; i < n; i++) {
+        fill_result(*ds[i], c->state_host[ds[i]->state_idx], &res[i]);
+        if (res[i].status != EP_OK) worst = res[i].status;
+    }
+    if (worst == EP_ERR_NONFINITE) fail(c, worst, "non-finite value in the placement state");
+    return worst;
+}
+
+int ep_place(ep_ctx* h, uint32_t id, const ep_config* cfg, ep_result* res) {
+    return ep_place_batch(h, &id, 1, cfg, res);
+}
+
+int ep_positions(ep_ctx* h, uint32_t id, double* x, double* y) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !x || !y) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    Design* D = find_design(c, id);
+    if (!D) return fail(c, EP_ERR_NOT_FOUND, "unknown design id");
+    const size_t N = (size_t)D->dev.n_mov;
+    if (N == 0) return EP_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int ax = 0; ax < 2; ax++) {
+        double* out = ax ? y : x;
+        if (D->prec == EP_PREC_FP64) {
+            HIPCHK(c, hipMemcpy(out, D->dev.pos[ax], N * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> f(N);
+            HIPCHK(c, hipMemcpy(f.data(), D->dev.pos[ax], N * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; i++) out[i] = f[i];
+        }
+    }
+    return EP_OK;
+}
+
+int ep_iter_log(ep_ctx* h, uint32_t id, ep_iter_rec* recs, uint32_t cap, uint32_t* n) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !n || (!recs && cap)) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    Design* D = find_design(c, id);
+    if (!D) return fail(c, EP_ERR_NOT_FOUND, "unknown design id");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    DevState s;
+    HIPCHK(c, hipMemcpy(&s, D->dev.st, sizeof(DevState), hipMemcpyDeviceToHost));
+    uint32_t k = (uint32_t)std::max(0, std::min(s.n_log, D->log_cap));
+    *n = k;
+    k = std::min(k, cap);
+    if (k) HIPCHK(c, hipMemcpy(recs, D->dev.log, k * sizeof(ep_iter_rec), hipMemcpyDeviceToHost));
+    return EP_OK;
+}
+
+}  // extern "C"
+
+// -------------------------------------------------------- evaluation -----
+
+namespace ep {
+
+// A single-design batch for the evaluation calls (context eval config).
+static int eval_prep(Ctx* c, uint32_t id, Design** Dout, Batch** bout) {
+    Design* D = find_design(c, id);
+    if (!D) return fail(c, EP_ERR_NOT_FOUND, "unknown design id");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<Design*> ds{D};
+    int rc;
+    Batch* b = cached_batch(c, ds, &c->eval_cfg, &rc);
+    if (!b) return rc;
+    DevState s;
+    memset(&s, 0, sizeof(s));
+    s.lam = 0.0;
+    s.a = 1.0;
+    c->state_host[D->state_idx] = s;
+    HIPCHK(c, hipMemcpyAsync(D->dev.st, &c->state_host[D->state_idx], sizeof(DevState),
+                             hipMemcpyHostToDevice, c->stream));
+    *Dout = D;
+    *bout = b;
+    return EP_OK;
+}
+
+static int put_T(Ctx* c, void* dst, const double* src, size_t n, int prec) {
+    if (!n) return EP_OK;
+    if (prec == EP_PREC_FP64) {
+        HIPCHK(c, hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> f(n);
+        for (size_t i = 0; i < n; i++) f[i] = (float)src[i];
+        HIPCHK(c, hipMemcpyAsync(dst, f.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return EP_OK;
+}
+
+static int get_T(Ctx* c, double* dst, const void* src, size_t n, int prec) {
+    if (!n) return EP_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (prec == EP_PREC_FP64) {
+        HIPCHK(c, hipMemcpy(dst, src, n * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> f(n);
+        HIPCHK(c, hipMemcpy(f.data(), src, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) dst[i] = f[i];
+    }
+    return EP_OK;
+}
+
+}  // namespace ep
+
+extern "C" {
+
+int ep_eval_config(ep_ctx* h, const ep_config* cfg) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !cfg) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    int rc = validate_config(c, cfg, kDefaultLogCap);
+    if (rc) return rc;
+    c->eval_cfg = *cfg;
+    release_batch_cache(c);
+    return EP_OK;
+}
+
+int ep_eval_wa_grad(ep_ctx* h, uint32_t id, const double* x, const double* y, double* gx, double* gy) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !x || !y || !gx || !gy) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    Design* D;
+    Batch* b;
+    int rc = eval_prep(c, id, &D, &b);
+    if (rc) return rc;
+    const size_t N = (size_t)D->dev.n_mov;
+    if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
+    if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
+    HIPCHK(c, hipMemsetAsync(D->dev.gwl[0], 0, N * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(D->dev.gwl[1], 0, N * 8, c->stream));
+    launch_wa(b->bd, b->prec, b->wa_blocks, b->hf_blocks, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int ax = 0; ax < 2; ax++) {
+        double* out = ax ? gy : gx;
+        if (!N) continue;
+        if (D->prec == EP_PREC_FP64) {
+            HIPCHK(c, hipMemcpy(out, D->dev.gwl[ax], N * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<long long> a(N);
+            HIPCHK(c, hipMemcpy(a.data(), D->dev.acc[ax], N * 8, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; i++) out[i] = (double)a[i] * (1.0 / kFixScale);
+        }
+    }
+    return EP_OK;
+}
+
+int ep_eval_density(ep_ctx* h, uint32_t id, const double* x, const double* y, double* dens,
+                    double* overflow) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !x || !y || !dens) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    Design* D;
+    Batch* b;
+    int rc = eval_prep(c, id, &D, &b);
+    if (rc) return rc;
+    const size_t N = (size_t)D->dev.n_mov, NB = (size_t)D->dev.Bx * D->dev.By;
+    if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
+    if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
+    launch_scatter(b->bd, b->prec, b->cell_blocks, c->stream);
+    launch_dens_final(b->bd, b->prec, b->bin_blocks, c->stream);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = get_T(c, dens, D->dev.dens, NB, D->prec))) return rc;
+    if (overflow) {
+        std::vector<double> p((size_t)D->dev.n_bin_blocks * kBinParts);
+        HIPCHK(c, hipMemcpy(p.data(), D->dev.part_bin, p.size() * 8, hipMemcpyDeviceToHost));
+        double tot = 0, ov = 0;
+        for (int k = 0; k < D->dev.n_bin_blocks; k++) {
+            tot += p[(size_t)k * kBinParts];
+            ov += p[(size_t)k * kBinParts + 1];
+        }
+        *overflow = tot < 1e-30 ? 0.0 : ov / tot;
+    }
+    return EP_OK;
+}
+
+int ep_eval_bin_ranges(ep_ctx* h, uint32_t id, const double* x, const double* y, int32_t* ranges) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !x || !y || !ranges) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    Design* D;
+    Batch* b;
+    int rc = eval_prep(c, id, &D, &b);
+    if (rc) return rc;
+    const size_t N = (size_t)D->dev.n_mov;
+    if (!N) return EP_OK;
+    if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
+    if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
+    int32_t* dbuf = nullptr;
+    HIPCHK(c, hipMalloc(&dbuf, N * 16));
+    launch_bin_ranges(b->bd, b->cell_blocks, dbuf, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(ranges, dbuf, N * 16, hipMemcpyDeviceToHost);
+    hipFree(dbuf);
+    if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, hipGetErrorString(e));
+    return EP_OK;
+}
+
+int ep_eval_field(ep_ctx* h, uint32_t id, const double* dens, double* ex, double* ey) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !dens || !ex || !ey) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    Design* D;
+    Batch* b;
+    int rc = eval_prep(c, id, &D, &b);
+    if (rc) return rc;
+    const size_t NB = (size_t)D->dev.Bx * D->dev.By;
+    if ((rc = put_T(c, D->dev.dens, dens, NB, D->prec))) return rc;
+    for (int p = 0; p < 4; p++)
+        for (int e = 0; e < 2; e++)
+            if (b->njobs2[p][e] > 0)
+                launch_gemm_pass(b->jobs2[p][e], b->njobs2[p][e], b->tiles2[p][e], b->d_dev, b->prec, e, c->stream);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = get_T(c, ex, D->dev.field[0], NB, D->prec))) return rc;
+    return get_T(c, ey, D->dev.field[1], NB, D->prec);
+}
+
+int ep_eval_density_grad(ep_ctx* h, uint32_t id, const double* x, const double* y, const double* ex,
+                         const double* ey, double* gx, double* gy) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !x || !y || !ex || !ey || !gx || !gy) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    Design* D;
+    Batch* b;
+    int rc = eval_prep(c, id, &D, &b);
+    if (rc) return rc;
+    const size_t N = (size_t)D->dev.n_mov, NB = (size_t)D->dev.Bx * D->dev.By;
+    if (!N) return EP_OK;
+    if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
+    if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
+    if ((rc = put_T(c, D->dev.field[0], ex, NB, D->prec))) return rc;
+    if ((rc = put_T(c, D->dev.field[1], ey, NB, D->prec))) return rc;
+    double* dbuf = nullptr;
+    HIPCHK(c, hipMalloc(&dbuf, N * 16));
+    launch_gather_export(b->bd, b->prec, b->cell_blocks, dbuf, dbuf + N, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(gx, dbuf, N * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(gy, dbuf + N, N * 8, hipMemcpyDeviceToHost);
+    hipFree(dbuf);
+    if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, hipGetErrorString(e));
+    return EP_OK;
+}
+
+int ep_eval_hpwl(ep_ctx* h, uint32_t id, const double* x, const double* y, double* hpwl) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !x || !y || !hpwl) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    Design* D;
+    Batch* b;
+    int rc = eval_prep(c, id, &D, &b);
+    if (rc) return rc;
+    const size_t N = (size_t)D->dev.n_mov;
+    if ((rc = put_T(c, D->dev.pos[0], x, N, D->prec))) return rc;
+    if ((rc = put_T(c, D->dev.pos[1], y, N, D->prec))) return rc;
+    launch_hpwl(b->bd, b->prec, b->wa_blocks, b->hf_blocks, 1, c->stream);
+    launch_ctrl(b->bd, 3, c->stream);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = read_states(c, *b))) return rc;
+    *hpwl = c->state_host[D->state_idx].final_hpwl;
+    return EP_OK;
+}
+
+}  // extern "C"

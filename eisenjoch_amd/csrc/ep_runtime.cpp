@@ -1,0 +1,495 @@
+// Host runtime of the MI355X placer: contexts, resident designs, batches,
+// the device-driven iteration loop and the single-kernel evaluation calls.
+//
+// A resident design is uploaded once into one device arena (SoA, 256-byte
+// aligned sub-buffers).  A batch is a list of resident designs plus the
+// device descriptors, block prefixes and transform jobs its kernels need;
+// one iteration of a batch is a fixed sequence of launches, captured once
+// into a hipGraph and replayed, with the host only reading back the small
+// FP64 state array between replays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ep_internal.h"
+#include "ep_runtime.h"
+
+namespace ep {
+
+// ------------------------------------------------------------- errors ----
+
+thread_local std::string g_tls_error;
+
+int fail(Ctx* c, int code, const std::string& msg) {
+    g_tls_error = msg;
+    if (c) {
+        std::lock_guard<std::mutex> g(c->err_mu);
+        c->last_error = msg;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                             \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(ctx, EP_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// ------------------------------------------------------- transforms ------
+
+// rustdct matrices (output index first): C2[k][j] = cos(pi k (2j+1) / 2n);
+// C3[j][k] = k ? cos(pi k (2j+1) / 2n) : 1/2; S3[j][k] = sin(pi (k+1)(2j+1) / 2n)
+// for k < n-1 and (-1)^j / 2 for k = n-1.  The IDXST mode replaces S3 by
+// S3'[j][k] = -sin(pi k (2j+1) / 2n) (0 for k = 0).
+static void build_matrix(int kind, int n, int mode, std::vector<double>& m) {
+    m.assign((size_t)n * n, 0.0);
+    const double pi = 3.14159265358979323846;
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < n; k++) {
+            double v;
+            if (kind == 0) v = std::cos(pi * j * (2.0 * k + 1.0) / (2.0 * n));  // C2[j][k]: j = freq
+            else if (kind == 1) v = k == 0 ? 0.5 : std::cos(pi * k * (2.0 * j + 1.0) / (2.0 * n));
+            else if (mode == kFieldReference)
+                v = k < n - 1 ? std::sin(pi * (k + 1.0) * (2.0 * j + 1.0) / (2.0 * n))
+                              : ((j & 1) ? -0.5 : 0.5);
+            else
+                v = k == 0 ? 0.0 : -std::sin(pi * k * (2.0 * j + 1.0) / (2.0 * n));
+            m[(size_t)j * n + k] = v;
+        }
+}
+
+static uint16_t bf16_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(r >> 16);
+}
+static float bf16_float(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+int get_transform(Ctx* c, int n, int mode, Transform** out) {
+    auto key = std::make_pair(n, mode);
+    auto it = c->transforms.find(key);
+    if (it != c->transforms.end()) {
+        *out = it->second.get();
+        return EP_OK;
+    }
+    std::unique_ptr<Transform> t(new Transform());
+    t->n = n;
+    t->mode = mode;
+    const size_t nn = (size_t)n * n;
+    const size_t bytes = 3 * align_up(nn * 8) + 3 * align_up(nn * 4) + 6 * align_up(nn * 2);
+    HIPCHK(c, hipMalloc(&t->mem, bytes));
+    char* p = (char*)t->mem;
+    std::vector<double> m;
+    std::vector<float> f(nn);
+    std::vector<uint16_t> hi(nn), lo(nn);
+    for (int kind = 0; kind < 3; kind++) {
+        build_matrix(kind, n, mode, m);
+        for (size_t i = 0; i < nn; i++) {
+            f[i] = (float)m[i];
+            hi[i] = bf16_bits(f[i]);
+            lo[i] = bf16_bits(f[i] - bf16_float(hi[i]));
+        }
+        t->d[kind] = (double*)p; p += align_up(nn * 8);
+        t->f[kind] = (float*)p; p += align_up(nn * 4);
+        t->hi[kind] = p; p += align_up(nn * 2);
+        t->lo[kind] = p; p += align_up(nn * 2);
+        HIPCHK(c, hipMemcpy(t->d[kind], m.data(), nn * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(t->f[kind], f.data(), nn * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(t->hi[kind], hi.data(), nn * 2, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(t->lo[kind], lo.data(), nn * 2, hipMemcpyHostToDevice));
+    }
+    c->transform_bytes += bytes;
+    *out = t.get();
+    c->transforms[key] = std::move(t);
+    return EP_OK;
+}
+
+// ------------------------------------------------------------ designs ----
+
+// Net layout: nets of degree 2..64 go to bucket b (group size G = 2^(b+1),
+// the smallest power of two >= degree); each net occupies G consecutive
+// slots; every bucket segment is padded to whole waves.  Larger nets go to a
+// CSR list handled one block per net.  Nets with fewer than two pins are
+// skipped, as common.rs:242 does.
+struct NetLayout {
+    std::vector<uint32_t> slots;
+    int n_buckets = kMaxBuckets;
+    int bucket_wave_start[kMaxBuckets + 1] = {0};
+    int bucket_group[kMaxBuckets] = {0};
+    std::vector<uint32_t> hf_off, hf_pins;
+};
+
+static void build_layout(const ep_design* d, NetLayout& L) {
+    std::vector<std::vector<uint32_t>> seg(kMaxBuckets);
+    L.hf_off.assign(1, 0);
+    for (int e = 0; e < d->n_nets; e++) {
+        uint32_t b0 = d->net_off[e], n = d->net_off[e + 1] - b0;
+        if (n < 2) continue;
+        if (n > 64) {
+            for (uint32_t k = 0; k < n; k++) L.hf_pins.push_back(d->pin_cell[b0 + k]);
+            L.hf_off.push_back((uint32_t)L.hf_pins.size());
+            continue;
+        }
+        int b = 0;
+        while ((2u << b) < n) b++;
+        const uint32_t G = 2u << b;
+        for (uint32_t k = 0; k < G; k++) seg[b].push_back(k < n ? d->pin_cell[b0 + k] : kPad);
+    }
+    int waves = 0;
+    for (int b = 0; b < kMaxBuckets; b++) {
+        L.bucket_wave_start[b] = waves;
+        L.bucket_group[b] = 2 << b;
+        while (seg[b].size() % kWave) seg[b].push_back(kPad);
+        waves += (int)(seg[b].size() / kWave);
+        L.slots.insert(L.slots.end(), seg[b].begin(), seg[b].end());
+    }
+    L.bucket_wave_start[kMaxBuckets] = waves;
+}
+
+static int validate_design(Ctx* c, const ep_design* d) {
+    if (!d) return fail(c, EP_ERR_INVALID_ARG, "null design");
+    if (d->n_mov < 0 || d->n_fixed < 0 || d->n_nets < 0 || d->n_pins < 0)
+        return fail(c, EP_ERR_INVALID_ARG, "negative size");
+    if (d->W < 1 || d->H < 1) return fail(c, EP_ERR_INVALID_ARG, "grid must be at least 1x1");
+    int bx = d->Bx > 0 ? d->Bx : d->W, by = d->By > 0 ? d->By : d->H;
+    if (bx < 1 || by < 1 || bx > 8192 || by > 8192)
+        return fail(c, EP_ERR_INVALID_ARG, "bin grid out of range");
+    if (d->n_mov > 0 && (!d->x0 || !d->y0)) return fail(c, EP_ERR_INVALID_ARG, "missing x0/y0");
+    if (d->n_fixed > 0 && (!d->fixed_x || !d->fixed_y))
+        return fail(c, EP_ERR_INVALID_ARG, "missing fixed coordinates");
+    if (d->n_nets > 0 && !d->net_off) return fail(c, EP_ERR_INVALID_ARG, "missing net_off");
+    if (d->n_nets > 0) {
+        if (d->net_off[0] != 0 || (int64_t)d->net_off[d->n_nets] != d->n_pins)
+            return fail(c, EP_ERR_INVALID_ARG, "net_off must start at 0 and end at n_pins");
+        for (int e = 0; e < d->n_nets; e++)
+            if (d->net_off[e + 1] < d->net_off[e])
+                return fail(c, EP_ERR_INVALID_ARG, "net_off not monotone");
+    }
+    if (d->n_pins > 0 && !d->pin_cell) return fail(c, EP_ERR_INVALID_ARG, "missing pin_cell");
+    const uint64_t ncell = (uint64_t)d->n_mov + (uint64_t)d->n_fixed;
+    for (int64_t p = 0; p < d->n_pins; p++)
+        if (d->pin_cell[p] >= ncell) return fail(c, EP_ERR_INVALID_ARG, "pin_cell out of range");
+    for (int i = 0; i < d->n_mov; i++)
+        if (!std::isfinite(d->x0[i]) || !std::isfinite(d->y0[i]))
+            return fail(c, EP_ERR_NONFINITE, "non-finite initial position");
+    if (d->area)
+        for (int i = 0; i < d->n_mov; i++)
+            if (!(d->area[i] > 0.0f) || !std::isfinite(d->area[i]))
+                return fail(c, EP_ERR_INVALID_ARG, "area must be positive and finite");
+    return EP_OK;
+}
+
+struct ArenaPlan {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        size_t o = off;
+        off += align_up(bytes > 0 ? bytes : 1);
+        return o;
+    }
+};
+
+// Computes the arena layout; with base == nullptr it only sizes it.
+static size_t plan_arena(Design& D, char* base) {
+    DesignDev& G = D.dev;
+    const size_t T = D.prec == EP_PREC_FP64 ? 8 : 4;
+    const size_t N = (size_t)G.n_mov, F = (size_t)G.n_fixed;
+    const size_t B = (size_t)G.Bx * G.By;
+    ArenaPlan a;
+    auto P = [&](size_t bytes) -> void* { size_t o = a.take(bytes); return base ? base + o : nullptr; };
+    for (int ax = 0; ax < 2; ax++) G.x0[ax] = P(N * T);
+    for (int ax = 0; ax < 2; ax++) G.pos[ax] = P(N * T);
+    for (int ax = 0; ax < 2; ax++) G.dpos[ax] = P(N * T);
+    for (int ax = 0; ax < 2; ax++) G.look[ax] = P(N * T);
+    for (int p = 0; p < 2; p++)
+        for (int ax = 0; ax < 2; ax++) G.g[p][ax] = P(N * T);
+    for (int ax = 0; ax < 2; ax++) {
+        void* q = P(N * 8);
+        G.gwl[ax] = q;
+        G.acc[ax] = (long long*)q;
+    }
+    G.pinw = P(N * T);
+    G.hs = (float*)P(N * 4);
+    for (int ax = 0; ax < 2; ax++) G.fixed[ax] = P(F * T);
+    G.slots = (uint32_t*)P((size_t)G.n_wa_waves * kWave * 4);
+    G.hf_off = (uint32_t*)P((size_t)(G.n_hf + 1) * 4);
+    G.hf_pins = (uint32_t*)P(D.hf_pin_count * 4);
+    G.dacc = P(B * 8);
+    G.dens = P(B * T);
+    G.tT = P(B * T);
+    for (int ax = 0; ax < 2; ax++) G.ehat[ax] = P(B * T);
+    for (int ax = 0; ax < 2; ax++) G.uT[ax] = P(B * T);
+    for (int ax = 0; ax < 2; ax++) G.field[ax] = P(B * T);
+    G.spec_x = P((size_t)2 * G.Bx * T);
+    G.spec_y = P((size_t)2 * G.By * T);
+    G.part_cell = (double*)P((size_t)G.n_cell_blocks * kCellParts * 8);
+    G.part_bin = (double*)P((size_t)G.n_bin_blocks * kBinParts * 8);
+    G.part_net = (double*)P((size_t)(G.n_wa_waves + G.n_hf + 1) * 8);
+    G.log = (ep_iter_rec*)P((size_t)D.log_cap * sizeof(ep_iter_rec));
+    return a.off;
+}
+
+template <typename T>
+static std::vector<T> conv(const double* p, size_t n) {
+    std::vector<T> v(n);
+    for (size_t i = 0; i < n; i++) v[i] = (T)p[i];
+    return v;
+}
+
+static int upload_T(Ctx* c, void* dst, const std::vector<double>& src, int prec) {
+    if (src.empty()) return EP_OK;
+    if (prec == EP_PREC_FP64) {
+        HIPCHK(c, hipMemcpy(dst, src.data(), src.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f = conv<float>(src.data(), src.size());
+        HIPCHK(c, hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    }
+    return EP_OK;
+}
+
+static void fill_desc_sizes(Design& D, const ep_design* d, const NetLayout& L) {
+    DesignDev& G = D.dev;
+    memset(&G, 0, sizeof(G));
+    G.n_mov = d->n_mov;
+    G.n_fixed = d->n_fixed;
+    G.n_nets = d->n_nets;
+    G.W = d->W;
+    G.H = d->H;
+    G.Bx = d->Bx > 0 ? d->Bx : d->W;
+    G.By = d->By > 0 ? d->By : d->H;
+    G.prec = D.prec;
+    G.n_cell_blocks = (G.n_mov + kBlock - 1) / kBlock;
+    G.n_bin_blocks = (G.Bx * G.By + kBlock - 1) / kBlock;
+    G.n_wa_waves = L.bucket_wave_start[kMaxBuckets];
+    G.n_hf = (int)L.hf_off.size() - 1;
+    G.n_buckets = kMaxBuckets;
+    for (int b = 0; b <= kMaxBuckets; b++) G.bucket_wave_start[b] = L.bucket_wave_start[b];
+    for (int b = 0; b < kMaxBuckets; b++) G.bucket_group[b] = L.bucket_group[b];
+    G.rx = (double)G.Bx / (double)G.W;
+    G.ry = (double)G.By / (double)G.H;
+    G.max_x = (double)(G.W - 1);  // mod.rs:73-74
+    G.max_y = (double)(G.H - 1);
+    D.hf_pin_count = L.hf_pins.size();
+}
+
+}  // namespace ep
+
+using namespace ep;
+
+extern "C" {
+
+int ep_version(void) { return 1; }
+
+void ep_config_default(ep_config* c) {
+    if (!c) return;
+    memset(c, 0, sizeof(*c));
+    c->seed = 1;  // config.rs:27-38
+    c->wl_coeff = 0.5;
+    c->target_util = 0.7;
+    c->timing_driven = 0;
+    c->max_iters = 500;
+    c->target_density = 1.0;
+    c->timing_weight = 0.0;
+    c->nesterov_step_size = 0.1;
+    c->legalize_interval = 5;
+    c->min_iters = 20;  // mod.rs:31-43
+    c->overflow_target = 0.1;
+    c->density_eta = 0.1;
+    c->grow_mult = 1.025;
+    c->grow_threshold = 50.0;
+    c->grow_add = 1.0;
+    c->stop_every_iter = 0;
+    c->log_hpwl = 1;
+    c->check_nonfinite = 1;
+    c->field_mode = EP_FIELD_REFERENCE;
+}
+
+int ep_create(ep_ctx** out, int device, uint64_t budget) {
+    if (!out) return fail(nullptr, EP_ERR_INVALID_ARG, "null out");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(nullptr, EP_ERR_DEVICE, "no HIP device available (the placer has no CPU path)");
+    if (device < 0 || device >= n) return fail(nullptr, EP_ERR_INVALID_ARG, "bad device index");
+    std::unique_ptr<Ctx> c(new Ctx());
+    c->device = device;
+    HIPCHK(nullptr, hipSetDevice(device));
+    size_t fr = 0, tot = 0;
+    HIPCHK(nullptr, hipMemGetInfo(&fr, &tot));
+    c->budget = budget ? budget : (uint64_t)(tot * 0.9);
+    HIPCHK(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(nullptr, hipMalloc(&c->state_pool, sizeof(DevState) * kStatePool));
+    HIPCHK(nullptr, hipHostMalloc(&c->state_host, sizeof(DevState) * kStatePool, hipHostMallocDefault));
+    for (int i = kStatePool - 1; i >= 0; i--) c->free_states.push_back(i);
+    ep_config_default(&c->eval_cfg);
+    c->t0 = now_ms();
+    *out = (ep_ctx*)c.release();
+    return EP_OK;
+}
+
+void ep_destroy(ep_ctx* h) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return;
+    ep_serve_stop(h);
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& kv : c->designs) {
+        hipFree(kv.second->arena);
+    }
+    c->designs.clear();
+    for (auto& kv : c->transforms) hipFree(kv.second->mem);
+    c->transforms.clear();
+    release_batch_cache(c);
+    hipFree(c->state_pool);
+    hipHostFree(c->state_host);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* ep_last_error(ep_ctx* h) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return g_tls_error.c_str();
+    std::lock_guard<std::mutex> g(c->err_mu);
+    c->err_copy = c->last_error;
+    return c->err_copy.c_str();
+}
+
+uint64_t ep_design_bytes(const ep_design* d, int32_t prec) {
+    if (!d) return 0;
+    NetLayout L;
+    build_layout(d, L);
+    Design D;
+    D.prec = prec == EP_PREC_FP64 ? EP_PREC_FP64 : EP_PREC_FAST;
+    D.log_cap = kDefaultLogCap;
+    fill_desc_sizes(D, d, L);
+    return (uint64_t)plan_arena(D, nullptr);
+}
+
+int ep_design_upload(ep_ctx* h, const ep_design* d, int32_t prec, uint32_t* id_out) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !id_out) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    int rc = validate_design(c, d);
+    if (rc) return rc;
+    if (prec != EP_PREC_FP64 && prec != EP_PREC_FAST)
+        return fail(c, EP_ERR_INVALID_ARG, "unknown precision");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    return upload_locked(c, d, prec, kDefaultLogCap, id_out);
+}
+
+}  // extern "C"
+
+namespace ep {
+
+static int fill_arena(Ctx* c, Design* D, const ep_design* d, const NetLayout& L) {
+    const int prec = D->prec;
+    const size_t N = (size_t)d->n_mov;
+    int rc;
+    std::vector<double> tmp(d->x0, d->x0 + N);
+    if ((rc = upload_T(c, D->dev.x0[0], tmp, prec))) return rc;
+    tmp.assign(d->y0, d->y0 + N);
+    if ((rc = upload_T(c, D->dev.x0[1], tmp, prec))) return rc;
+    tmp.assign(d->fixed_x, d->fixed_x + d->n_fixed);
+    if ((rc = upload_T(c, D->dev.fixed[0], tmp, prec))) return rc;
+    tmp.assign(d->fixed_y, d->fixed_y + d->n_fixed);
+    if ((rc = upload_T(c, D->dev.fixed[1], tmp, prec))) return rc;
+    // pin weights, common.rs:381-407 (or the caller's)
+    std::vector<double> w(N, 0.0);
+    if (d->pinw) {
+        w.assign(d->pinw, d->pinw + N);
+    } else {
+        for (int e = 0; e < d->n_nets; e++) {
+            uint32_t b0 = d->net_off[e], n = d->net_off[e + 1] - b0;
+            if (n < 2) continue;
+            double wt = 1.0 / (double)n;
+            for (uint32_t k = 0; k < n; k++) {
+                uint32_t cc = d->pin_cell[b0 + k];
+                if (cc < (uint32_t)d->n_mov) w[cc] += wt;
+            }
+        }
+    }
+    if ((rc = upload_T(c, D->dev.pinw, w, prec))) return rc;
+    std::vector<float> hs(N);
+    for (size_t i = 0; i < N; i++)
+        hs[i] = (float)(0.5 * std::sqrt((double)(d->area ? d->area[i] : 1.0f)));
+    if (N) HIPCHK(c, hipMemcpy(D->dev.hs, hs.data(), N * 4, hipMemcpyHostToDevice));
+    if (!L.slots.empty())
+        HIPCHK(c, hipMemcpy(D->dev.slots, L.slots.data(), L.slots.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(D->dev.hf_off, L.hf_off.data(), L.hf_off.size() * 4, hipMemcpyHostToDevice));
+    if (!L.hf_pins.empty())
+        HIPCHK(c, hipMemcpy(D->dev.hf_pins, L.hf_pins.data(), L.hf_pins.size() * 4, hipMemcpyHostToDevice));
+    const double pi = 3.14159265358979323846;
+    std::vector<double> sx(2 * (size_t)D->dev.Bx), sy(2 * (size_t)D->dev.By);
+    for (int k = 0; k < D->dev.Bx; k++) {  // density.rs:139-140
+        double w0 = pi * k / (double)D->dev.Bx;
+        sx[2 * k] = w0;
+        sx[2 * k + 1] = 2.0 * (1.0 - std::cos(w0));
+    }
+    for (int k = 0; k < D->dev.By; k++) {  // density.rs:135-136
+        double w0 = pi * k / (double)D->dev.By;
+        sy[2 * k] = w0;
+        sy[2 * k + 1] = 2.0 * (1.0 - std::cos(w0));
+    }
+    if ((rc = upload_T(c, D->dev.spec_x, sx, prec))) return rc;
+    if ((rc = upload_T(c, D->dev.spec_y, sy, prec))) return rc;
+    return EP_OK;
+}
+
+int upload_locked(Ctx* c, const ep_design* d, int prec, int log_cap, uint32_t* id_out) {
+    HIPCHK(c, hipSetDevice(c->device));
+    NetLayout L;
+    build_layout(d, L);
+    std::unique_ptr<Design> D(new Design());
+    D->prec = prec;
+    D->log_cap = log_cap;
+    fill_desc_sizes(*D, d, L);
+    const size_t bytes = plan_arena(*D, nullptr);
+    if (c->used + bytes > c->budget)
+        return fail(c, EP_ERR_OOM, "design does not fit in the HBM budget");
+    if (c->free_states.empty()) return fail(c, EP_ERR_OOM, "state pool exhausted");
+    hipError_t e = hipMalloc(&D->arena, bytes);
+    if (e != hipSuccess) return fail(c, EP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    D->bytes = bytes;
+    plan_arena(*D, (char*)D->arena);
+    e = hipMemset(D->arena, 0, bytes);
+    int rc = e == hipSuccess ? fill_arena(c, D.get(), d, L)
+                             : fail(c, EP_ERR_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
+    if (rc) {
+        hipFree(D->arena);
+        return rc;
+    }
+    D->state_idx = c->free_states.back();
+    c->free_states.pop_back();
+    D->dev.st = c->state_pool + D->state_idx;
+    D->id = c->next_id++;
+    c->used += bytes;
+    *id_out = D->id;
+    c->designs[D->id] = std::move(D);
+    return EP_OK;
+}
+
+}  // namespace ep

@@ -1,0 +1,110 @@
+// Host-side objects of the placer runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ep_internal.h"
+
+struct ep_ctx {};
+
+namespace ep {
+
+constexpr int kStatePool = 8192;      // resident designs per context
+constexpr int kDefaultLogCap = 2048;  // iteration records per design
+
+// Constant transform matrices of one size and field mode: [0] C2, [1] C3,
+// [2] S3 (or the IDXST sine matrix), each in FP64, FP32 and BF16 hi/lo.
+struct Transform {
+    int n = 0, mode = 0;
+    void* mem = nullptr;
+    double* d[3] = {nullptr, nullptr, nullptr};
+    float* f[3] = {nullptr, nullptr, nullptr};
+    void* hi[3] = {nullptr, nullptr, nullptr};
+    void* lo[3] = {nullptr, nullptr, nullptr};
+};
+
+struct Design {
+    uint32_t id = 0;
+    int prec = EP_PREC_FAST;
+    int log_cap = kDefaultLogCap;
+    int state_idx = -1;
+    void* arena = nullptr;
+    size_t bytes = 0;
+    size_t hf_pin_count = 0;
+    DesignDev dev{};
+};
+
+// A batch of resident designs sharing every launch of an iteration.  The
+// transform jobs of each pass are split by engine ([pass][0] MFMA, [pass][1]
+// FMA) per design shape, so a design takes the same path in any batch.
+struct Batch {
+    std::vector<Design*> ds;
+    std::vector<ep_config> cfgs;
+    int prec = EP_PREC_FAST;
+    int log_hpwl = 1;
+    int cell_blocks = 0, bin_blocks = 0, wa_blocks = 0, hf_blocks = 0;
+    int njobs2[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    int tiles2[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    GemmJob* jobs2[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    void* mem = nullptr;        // descriptors, prefixes, jobs
+    DesignDev* d_dev = nullptr;
+    const int32_t* dzero = nullptr;  // a device zero: prefix of a one-design view
+    BatchDev bd{};
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 0;
+};
+
+struct Server;
+struct ServerDeleter {
+    void operator()(Server* s) const;
+};
+
+struct Ctx : ep_ctx {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    uint64_t transform_bytes = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;     // serialises device work
+    std::mutex err_mu;
+    std::string last_error, err_copy;
+    std::map<uint32_t, std::unique_ptr<Design>> designs;
+    std::map<std::pair<int, int>, std::unique_ptr<Transform>> transforms;
+    DevState* state_pool = nullptr;
+    DevState* state_host = nullptr;  // pinned mirror
+    std::vector<int> free_states;
+    uint32_t next_id = 1;
+    double t0 = 0.0;
+    std::unique_ptr<Batch> cache;    // last built batch (graph kept for replays)
+    ep_config cache_cfg{};
+    bool cache_cfg_valid = false;
+    ep_config eval_cfg{};            // configuration of the ep_eval_* calls
+    bool server_running = false;
+    std::unique_ptr<Server, ServerDeleter> server;
+};
+
+int fail(Ctx* c, int code, const std::string& msg);
+double now_ms();
+int get_transform(Ctx* c, int n, int mode, Transform** out);
+int upload_locked(Ctx* c, const ep_design* d, int prec, int log_cap, uint32_t* id_out);
+int free_design_locked(Ctx* c, uint32_t id);
+void release_batch_cache(Ctx* c);
+
+// Batches (ep_place.cpp)
+int batch_build(Ctx* c, const std::vector<Design*>& ds, const ep_config* cfg, Batch& b);
+void batch_release(Batch& b);
+int batch_reset(Ctx* c, Batch& b, const ep_config* cfg);
+int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph);
+int batch_finalize(Ctx* c, Batch& b);
+int read_states(Ctx* c, const Batch& b);
+void fill_result(const Design& D, const DevState& s, ep_result* r);
+
+}  // namespace ep

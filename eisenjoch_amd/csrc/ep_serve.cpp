@@ -1,0 +1,354 @@
+// Serving: continuous batching of placement requests on one GPU.
+//
+// Requests are queued by ep_serve_submit (the design is copied), admitted in
+// arrival order while a slot is free and the design's arena fits in the HBM
+// budget, and then advance inside the running batch: every fused iteration
+// launch covers all admitted designs, each at its own iteration count.  A
+// design leaves the batch when its own state says it is done; its slot and
+// memory are reused by the next admission.  Because the FAST path's sums are
+// fixed-point and all reductions are per design in a fixed order, a design's
+// trajectory does not depend on which other designs share its launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ep_internal.h"
+#include "ep_runtime.h"
+
+namespace ep {
+
+int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<const ep_config*>& cfgs,
+                     Batch& b);
+int batch_reset_some(Ctx* c, Batch& b, const std::vector<int>& idx);
+int validate_config(Ctx* c, const ep_config* cfg, int log_cap);
+
+enum ReqState { kQueued = 0, kRunning = 1, kDone = 2 };
+
+struct ServeReq {
+    uint64_t ticket = 0;
+    std::vector<double> x0, y0, fx, fy, pinw;
+    std::vector<float> area;
+    std::vector<uint32_t> off, pins;
+    ep_design d{};
+    ep_config cfg{};
+    int state = kQueued;
+    int status = EP_PENDING;
+    bool cancel = false;
+    ep_result res{};
+    std::vector<double> xo, yo;
+    Design* D = nullptr;
+    uint64_t bytes = 0;
+};
+
+struct Server {
+    std::thread th;
+    std::mutex m;
+    std::condition_variable cv;
+    bool stop = false;
+    ep_serve_config scfg{};
+    std::deque<ServeReq*> queue;
+    std::vector<ServeReq*> active;
+    std::map<uint64_t, std::unique_ptr<ServeReq>> all;
+    uint64_t next_ticket = 1;
+    Batch batch;
+    bool dirty = true;
+};
+
+void ServerDeleter::operator()(Server* s) const { delete s; }
+
+static BatchDev single_view(const Batch& b, int i) {
+    BatchDev one = b.bd;
+    one.d = b.d_dev + i;
+    one.nd = 1;
+    one.pfx_cell = one.pfx_bin = one.pfx_wa = one.pfx_hf = b.dzero;
+    return one;
+}
+
+// Reduce pending log partials, compute the final HPWL and read the state of
+// design i of the batch, leaving the others untouched.
+static int finalize_one(Ctx* c, Batch& b, int i) {
+    const Design& D = *b.ds[i];
+    BatchDev one = single_view(b, i);
+    launch_ctrl(one, 2, c->stream);
+    launch_hpwl(one, b.prec, (D.dev.n_wa_waves + 3) / 4, D.dev.n_hf, 1, c->stream);
+    launch_ctrl(one, 3, c->stream);
+    hipError_t e = hipMemcpyAsync(&c->state_host[D.state_idx], D.dev.st, sizeof(DevState),
+                                  hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, hipGetErrorString(e));
+    return EP_OK;
+}
+
+static int copy_positions(Ctx* c, const Design& D, std::vector<double>& x, std::vector<double>& y) {
+    const size_t N = (size_t)D.dev.n_mov;
+    x.resize(N);
+    y.resize(N);
+    if (!N) return EP_OK;
+    for (int ax = 0; ax < 2; ax++) {
+        double* out = ax ? y.data() : x.data();
+        hipError_t e;
+        if (D.prec == EP_PREC_FP64) {
+            e = hipMemcpy(out, D.dev.pos[ax], N * 8, hipMemcpyDeviceToHost);
+        } else {
+            std::vector<float> f(N);
+            e = hipMemcpy(f.data(), D.dev.pos[ax], N * 4, hipMemcpyDeviceToHost);
+            for (size_t k = 0; k < N; k++) out[k] = f[k];
+        }
+        if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, hipGetErrorString(e));
+    }
+    return EP_OK;
+}
+
+static void serve_loop(Ctx* c) {
+    Server& S = *c->server;
+    hipSetDevice(c->device);
+    std::vector<int> fresh;  // batch indices admitted since the last rebuild
+    std::vector<ServeReq*> fresh_reqs;
+    for (;;) {
+        std::vector<ServeReq*> admit;
+        {
+            std::unique_lock<std::mutex> lk(S.m);
+            if (S.stop) break;
+            // drop cancelled requests still in the queue
+            for (auto it = S.queue.begin(); it != S.queue.end();) {
+                if ((*it)->cancel) {
+                    (*it)->state = kDone;
+                    (*it)->status = EP_ERR_CANCELLED;
+                    (*it)->res.status = EP_ERR_CANCELLED;
+                    (*it)->res.t_done_ms = now_ms() - c->t0;
+                    it = S.queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            uint64_t free_b = c->budget > c->used ? c->budget - c->used : 0;
+            uint64_t pending = 0;
+            while (!S.queue.empty() && S.active.size() + admit.size() < S.scfg.max_slots) {
+                ServeReq* r = S.queue.front();
+                if (pending + r->bytes > free_b) break;  // arrival order: no overtaking
+                pending += r->bytes;
+                admit.push_back(r);
+                S.queue.pop_front();
+            }
+            if (admit.empty() && S.active.empty()) {
+                S.cv.wait_for(lk, std::chrono::milliseconds(2));
+                continue;
+            }
+        }
+        std::lock_guard<std::recursive_mutex> g(c->mu);
+        for (ServeReq* r : admit) {
+            uint32_t id = 0;
+            int rc = upload_locked(c, &r->d, EP_PREC_FAST, std::max(1, r->cfg.max_iters), &id);
+            std::lock_guard<std::mutex> lk(S.m);
+            r->res.t_admit_ms = now_ms() - c->t0;
+            if (rc) {
+                r->state = kDone;
+                r->status = rc;
+                r->res.status = rc;
+                r->res.t_done_ms = r->res.t_admit_ms;
+                continue;
+            }
+            r->D = c->designs[id].get();
+            r->state = kRunning;
+            S.active.push_back(r);
+            fresh_reqs.push_back(r);
+            S.dirty = true;
+        }
+        if (S.active.empty()) continue;
+        if (S.dirty) {
+            std::vector<Design*> ds;
+            std::vector<const ep_config*> cf;
+            for (ServeReq* r : S.active) {
+                ds.push_back(r->D);
+                cf.push_back(&r->cfg);
+            }
+            if (batch_build_cfgs(c, ds, cf, S.batch)) break;
+            fresh.clear();
+            for (size_t i = 0; i < S.active.size(); i++)
+                if (std::find(fresh_reqs.begin(), fresh_reqs.end(), S.active[i]) != fresh_reqs.end())
+                    fresh.push_back((int)i);
+            if (!fresh.empty() && batch_reset_some(c, S.batch, fresh)) break;
+            fresh_reqs.clear();
+            S.dirty = false;
+        }
+        if (batch_run_iters(c, S.batch, S.scfg.iters_per_sync, getenv("EP_NO_GRAPH") == nullptr)) break;
+        if (read_states(c, S.batch)) break;
+        bool changed = false;
+        for (int i = 0; i < (int)S.active.size(); i++) {
+            ServeReq* r = S.active[i];
+            const DevState& st = c->state_host[r->D->state_idx];
+            bool cancel;
+            {
+                std::lock_guard<std::mutex> lk(S.m);
+                cancel = r->cancel;
+            }
+            if (!st.done && !cancel) continue;
+            finalize_one(c, S.batch, i);
+            std::vector<double> xo, yo;
+            copy_positions(c, *r->D, xo, yo);
+            ep_result res = r->res;
+            fill_result(*r->D, c->state_host[r->D->state_idx], &res);
+            if (cancel) res.status = EP_ERR_CANCELLED;
+            res.t_done_ms = now_ms() - c->t0;
+            free_design_locked(c, r->D->id);
+            std::lock_guard<std::mutex> lk(S.m);
+            r->D = nullptr;
+            r->xo.swap(xo);
+            r->yo.swap(yo);
+            r->res = res;
+            r->status = res.status;
+            r->state = kDone;
+            S.active[i] = nullptr;
+            changed = true;
+        }
+        if (changed) {
+            std::lock_guard<std::mutex> lk(S.m);
+            S.active.erase(std::remove(S.active.begin(), S.active.end(), nullptr), S.active.end());
+            S.dirty = true;
+            if (S.active.empty()) batch_release(S.batch);
+        }
+    }
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    batch_release(S.batch);
+}
+
+}  // namespace ep
+
+using namespace ep;
+
+extern "C" {
+
+int ep_serve_start(ep_ctx* h, const ep_serve_config* scfg) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return fail(nullptr, EP_ERR_INVALID_ARG, "null context");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    if (c->server_running) return fail(c, EP_ERR_INVALID_ARG, "already serving");
+    release_batch_cache(c);
+    c->server.reset(new Server());
+    ep_serve_config s{128, 5, 0};
+    if (scfg) s = *scfg;
+    if (s.max_slots == 0 || s.max_slots > 128) return fail(c, EP_ERR_INVALID_ARG, "max_slots must be 1..128");
+    if (s.iters_per_sync == 0) s.iters_per_sync = 5;
+    if (s.hbm_budget_bytes) c->budget = std::min<uint64_t>(c->budget, s.hbm_budget_bytes);
+    c->server->scfg = s;
+    c->server_running = true;
+    c->server->th = std::thread(serve_loop, c);
+    return EP_OK;
+}
+
+int ep_serve_stop(ep_ctx* h) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !c->server) return EP_OK;
+    {
+        std::lock_guard<std::mutex> lk(c->server->m);
+        c->server->stop = true;
+    }
+    c->server->cv.notify_all();
+    if (c->server->th.joinable()) c->server->th.join();
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    for (ServeReq* r : c->server->active)
+        if (r && r->D) free_design_locked(c, r->D->id);
+    c->server.reset();
+    c->server_running = false;
+    return EP_OK;
+}
+
+int ep_serve_submit(ep_ctx* h, const ep_design* d, const ep_config* cfg, uint64_t* ticket) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !d || !cfg || !ticket) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    if (!c->server_running) return fail(c, EP_ERR_INVALID_ARG, "serving not started");
+    int rc = validate_config(c, cfg, 1 << 20);
+    if (rc) return rc;
+    std::unique_ptr<ServeReq> r(new ServeReq());
+    const size_t N = (size_t)std::max(0, d->n_mov), F = (size_t)std::max(0, d->n_fixed);
+    r->x0.assign(d->x0, d->x0 + N);
+    r->y0.assign(d->y0, d->y0 + N);
+    if (F) {
+        r->fx.assign(d->fixed_x, d->fixed_x + F);
+        r->fy.assign(d->fixed_y, d->fixed_y + F);
+    }
+    if (d->area) r->area.assign(d->area, d->area + N);
+    if (d->pinw) r->pinw.assign(d->pinw, d->pinw + N);
+    if (d->net_off) r->off.assign(d->net_off, d->net_off + d->n_nets + 1);
+    if (d->n_pins) r->pins.assign(d->pin_cell, d->pin_cell + d->n_pins);
+    r->d = *d;
+    r->d.x0 = r->x0.data();
+    r->d.y0 = r->y0.data();
+    r->d.fixed_x = F ? r->fx.data() : nullptr;
+    r->d.fixed_y = F ? r->fy.data() : nullptr;
+    r->d.area = d->area ? r->area.data() : nullptr;
+    r->d.pinw = d->pinw ? r->pinw.data() : nullptr;
+    r->d.net_off = d->net_off ? r->off.data() : nullptr;
+    r->d.pin_cell = d->n_pins ? r->pins.data() : nullptr;
+    r->cfg = *cfg;
+    r->bytes = ep_design_bytes(&r->d, EP_PREC_FAST);
+    if (r->bytes > c->budget) return fail(c, EP_ERR_ADMISSION_REFUSED, "design exceeds the HBM budget");
+    Server& S = *c->server;
+    std::lock_guard<std::mutex> lk(S.m);
+    r->ticket = S.next_ticket++;
+    r->res.status = EP_PENDING;
+    r->res.t_submit_ms = now_ms() - c->t0;
+    *ticket = r->ticket;
+    S.queue.push_back(r.get());
+    S.all[r->ticket] = std::move(r);
+    S.cv.notify_all();
+    return EP_OK;
+}
+
+int ep_serve_poll(ep_ctx* h, uint64_t ticket, ep_result* res) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !c->server) return fail(c, EP_ERR_INVALID_ARG, "serving not started");
+    Server& S = *c->server;
+    std::lock_guard<std::mutex> lk(S.m);
+    auto it = S.all.find(ticket);
+    if (it == S.all.end()) return fail(c, EP_ERR_NOT_FOUND, "unknown ticket");
+    if (res) *res = it->second->res;
+    if (it->second->state != kDone) return EP_PENDING;
+    return it->second->status;
+}
+
+int ep_serve_positions(ep_ctx* h, uint64_t ticket, double* x, double* y) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !c->server || !x || !y) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    Server& S = *c->server;
+    std::lock_guard<std::mutex> lk(S.m);
+    auto it = S.all.find(ticket);
+    if (it == S.all.end()) return fail(c, EP_ERR_NOT_FOUND, "unknown ticket");
+    if (it->second->state != kDone) return EP_PENDING;
+    if (it->second->xo.empty() && it->second->d.n_mov > 0)
+        return fail(c, it->second->status ? it->second->status : EP_ERR_NOT_FOUND, "no positions");
+    std::copy(it->second->xo.begin(), it->second->xo.end(), x);
+    std::copy(it->second->yo.begin(), it->second->yo.end(), y);
+    return EP_OK;
+}
+
+int ep_serve_release(ep_ctx* h, uint64_t ticket) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !c->server) return fail(c, EP_ERR_INVALID_ARG, "serving not started");
+    Server& S = *c->server;
+    std::lock_guard<std::mutex> lk(S.m);
+    auto it = S.all.find(ticket);
+    if (it == S.all.end()) return fail(c, EP_ERR_NOT_FOUND, "unknown ticket");
+    if (it->second->state != kDone) return fail(c, EP_ERR_INVALID_ARG, "request still pending");
+    S.all.erase(it);
+    return EP_OK;
+}
+
+int ep_serve_cancel(ep_ctx* h, uint64_t ticket) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !c->server) return fail(c, EP_ERR_INVALID_ARG, "serving not started");
+    Server& S = *c->server;
+    std::lock_guard<std::mutex> lk(S.m);
+    auto it = S.all.find(ticket);
+    if (it == S.all.end()) return fail(c, EP_ERR_NOT_FOUND, "unknown ticket");
+    it->second->cancel = true;
+    S.cv.notify_all();
+    return EP_OK;
+}
+
+}  // extern "C"

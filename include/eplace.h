@@ -1,0 +1,228 @@
+/*
+ * eplace.h -- C ABI of the MI355X electrostatic global placer.
+ *
+ * This library is the device half of the reference's ElectroPlace placer
+ * (crates/nextpnr/src/placer/electro_place/mod.rs:65-204, `place_electro`).
+ * The reference selects it by the string "electro" in the PyO3 `place()`
+ * match (crates/npnr-python/src/lib.rs:293-302) and through the `Placer`
+ * trait (crates/nextpnr/src/placer/mod.rs:31-52).  A host that owns the
+ * netlist flattens it into an `ep_design` (solver-index order, nets in slot
+ * order, driver pin first) and calls `ep_place`; legalization and BEL
+ * binding stay on the host (`ep_legalize` offers the reference's greedy
+ * rule).  Plain pointers and sizes only; every buffer is caller-owned and
+ * copied.  All entry points are thread-safe per context.
+ */
+#ifndef EPLACE_H
+#define EPLACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  -2 and -3 mirror PlacerError::NoBelsAvailable and
+ * PlacerError::PlacementFailed (crates/nextpnr/src/placer/mod.rs:20-28). */
+#define EP_OK 0
+#define EP_PENDING 1
+#define EP_ERR_INVALID_ARG (-1)
+#define EP_ERR_NO_BELS (-2)
+#define EP_ERR_PLACEMENT_FAILED (-3)
+#define EP_ERR_OOM (-4)
+#define EP_ERR_ADMISSION_REFUSED (-5)
+#define EP_ERR_DEVICE (-6)
+#define EP_ERR_NONFINITE (-7)
+#define EP_ERR_NOT_FOUND (-8)
+#define EP_ERR_CANCELLED (-9)
+
+/* Arithmetic of a resident design. */
+#define EP_PREC_FP64 0 /* plain placer: FP64 everywhere, FP64 dense transforms */
+#define EP_PREC_FAST 1 /* FP32 bulk, BF16 hi+lo MFMA transforms, FP64 reductions */
+
+/* Field of the spectral solve.  REFERENCE feeds the DCT-indexed spectrum
+ * straight into DST-III exactly as density.rs:155-163 does (the sine of
+ * frequency k+1 carries coefficient k).  IDXST shifts it by one and negates
+ * it, the textbook field -d(phi)/dx whose force spreads cells; it is the only
+ * departure from the reference and exists because the reference field does
+ * not reach overflow <= 0.10 on the synthetic designs. */
+#define EP_FIELD_REFERENCE 0
+#define EP_FIELD_IDXST 1
+
+typedef struct ep_ctx ep_ctx;
+
+/* A flattened placement problem.  Movable cells are 0..n_mov-1 in solver
+ * index order (common.rs:155-174); pin ids >= n_mov name fixed cells
+ * (id - n_mov).  Each net's pins are its collected pins, driver first
+ * (common.rs:228-240).  Bx/By = 0 means bins = tiles (the reference,
+ * mod.rs:75-76).  area = NULL means unit cells (the reference); pinw =
+ * NULL derives common.rs:381-407 weights from the CSR degrees. */
+typedef struct ep_design {
+    int32_t n_mov;
+    int32_t n_fixed;
+    int32_t n_nets;
+    int32_t W, H;
+    int32_t Bx, By;
+    int32_t reserved;
+    int64_t n_pins;
+    const double* x0;       /* [n_mov] initial positions (init_positions_from_bels) */
+    const double* y0;       /* [n_mov] */
+    const float* area;      /* [n_mov] or NULL */
+    const double* fixed_x;  /* [n_fixed] */
+    const double* fixed_y;  /* [n_fixed] */
+    const uint32_t* net_off;   /* [n_nets + 1] */
+    const uint32_t* pin_cell;  /* [n_pins] */
+    const double* pinw;        /* [n_mov] or NULL */
+} ep_design;
+
+/* ElectroPlaceCfg (electro_place/config.rs:5-40) plus the constants of
+ * mod.rs:31-43 and this library's knobs.  ep_config_default() gives the
+ * reference defaults. */
+typedef struct ep_config {
+    uint64_t seed;              /* 1 */
+    double wl_coeff;            /* 0.5 */
+    double target_util;         /* 0.7 (unused by the reference loop) */
+    int32_t timing_driven;      /* 0 (unused by the reference loop) */
+    int32_t max_iters;          /* 500 */
+    double target_density;      /* 1.0 (only moves the zeroed DC term) */
+    double timing_weight;       /* 0.0 (unused) */
+    double nesterov_step_size;  /* 0.1 */
+    int32_t legalize_interval;  /* 5 */
+    int32_t min_iters;          /* 20  (CONVERGENCE_MIN_ITERS) */
+    double overflow_target;     /* 0.1 (OVERLAP_CONVERGE) */
+    double density_eta;         /* 0.1 (DENSITY_ETA) */
+    double grow_mult;           /* 1.025 */
+    double grow_threshold;      /* 50.0 */
+    double grow_add;            /* 1.0 */
+    int32_t stop_every_iter;    /* 0: reference rule, check on legalize iterations */
+    int32_t log_hpwl;           /* 1: continuous HPWL into the log on check iterations */
+    int32_t check_nonfinite;    /* 1: fail with EP_ERR_NONFINITE on NaN/Inf */
+    int32_t field_mode;         /* EP_FIELD_REFERENCE or EP_FIELD_IDXST */
+} ep_config;
+
+/* Per-iteration record (mod.rs:185-188 logs the same quantities). */
+typedef struct ep_iter_rec {
+    int32_t iter;
+    int32_t checked;        /* convergence-check iteration */
+    double overflow;        /* compute_overlap at the look-ahead point */
+    double lambda;          /* density penalty used this iteration */
+    double step_size_x, step_size_y;
+    double step_norm_x, step_norm_y;
+    double hpwl;            /* continuous HPWL after the step (check iterations) */
+    double wl_norm, den_norm; /* norms used by the penalty initialisation */
+} ep_iter_rec;
+
+typedef struct ep_result {
+    int32_t status;
+    int32_t iters;          /* iterations executed */
+    int32_t converged;      /* overflow < target on a check iteration */
+    int32_t precision;
+    double overflow;        /* overflow of the last executed iteration */
+    double hpwl;            /* continuous HPWL of the final positions (FP64) */
+    double lambda;
+    double t_submit_ms;     /* serving: submit, admit and finish times */
+    double t_admit_ms;
+    double t_done_ms;
+} ep_result;
+
+/* ---- context ---- */
+int ep_create(ep_ctx** out, int device, uint64_t hbm_budget_bytes);
+void ep_destroy(ep_ctx* ctx);
+const char* ep_last_error(ep_ctx* ctx);
+void ep_config_default(ep_config* cfg);
+int ep_version(void);
+
+/* ---- resident designs ---- */
+int ep_design_upload(ep_ctx* ctx, const ep_design* d, int32_t precision, uint32_t* design_id);
+int ep_design_free(ep_ctx* ctx, uint32_t design_id);
+uint64_t ep_design_bytes(const ep_design* d, int32_t precision);
+
+/* Runs place_electro's continuous loop (mod.rs:104-197) on one resident
+ * design, synchronously. */
+int ep_place(ep_ctx* ctx, uint32_t design_id, const ep_config* cfg, ep_result* res);
+/* Advances every listed design in the same launches until each stops. */
+int ep_place_batch(ep_ctx* ctx, const uint32_t* ids, uint32_t n, const ep_config* cfg,
+                   ep_result* results);
+int ep_positions(ep_ctx* ctx, uint32_t design_id, double* x, double* y);
+int ep_iter_log(ep_ctx* ctx, uint32_t design_id, ep_iter_rec* recs, uint32_t cap, uint32_t* n);
+
+/* ---- single-kernel evaluation on a resident design (host buffers) ---- */
+/* Configuration used by the ep_eval_* calls (wl_coeff, field_mode); the
+ * reference defaults until set. */
+int ep_eval_config(ep_ctx* ctx, const ep_config* cfg);
+/* WA gradient (common.rs:205-265, wa.rs:51-101) at positions (x, y). */
+int ep_eval_wa_grad(ep_ctx* ctx, uint32_t id, const double* x, const double* y,
+                    double* gx, double* gy);
+/* Density (density.rs:54-74) and overflow (density.rs:79-86). */
+int ep_eval_density(ep_ctx* ctx, uint32_t id, const double* x, const double* y,
+                    double* dens, double* overflow);
+/* Integer bin ranges {bx_lo, bx_hi, by_lo, by_hi} per cell (density.rs:20-37). */
+int ep_eval_bin_ranges(ep_ctx* ctx, uint32_t id, const double* x, const double* y,
+                       int32_t* ranges);
+/* Field (density.rs:96-166) of a given density grid [By*Bx]. */
+int ep_eval_field(ep_ctx* ctx, uint32_t id, const double* dens, double* ex, double* ey);
+/* Density gradient gather (density.rs:171-206). */
+int ep_eval_density_grad(ep_ctx* ctx, uint32_t id, const double* x, const double* y,
+                         const double* ex, const double* ey, double* gx, double* gy);
+/* Continuous HPWL (FP64 reduction). */
+int ep_eval_hpwl(ep_ctx* ctx, uint32_t id, const double* x, const double* y, double* hpwl);
+
+/* ---- serving: continuous batching under an HBM budget ---- */
+typedef struct ep_serve_config {
+    uint32_t max_slots;         /* designs advanced per fused iteration (<= 128) */
+    uint32_t iters_per_sync;    /* iterations between host polls (5) */
+    uint64_t hbm_budget_bytes;  /* 0: the context budget */
+} ep_serve_config;
+
+int ep_serve_start(ep_ctx* ctx, const ep_serve_config* scfg);
+int ep_serve_submit(ep_ctx* ctx, const ep_design* d, const ep_config* cfg, uint64_t* ticket);
+/* 0 (EP_OK) when done, EP_PENDING while queued or running, < 0 on error. */
+int ep_serve_poll(ep_ctx* ctx, uint64_t ticket, ep_result* res);
+int ep_serve_positions(ep_ctx* ctx, uint64_t ticket, double* x, double* y);
+int ep_serve_release(ep_ctx* ctx, uint64_t ticket);
+int ep_serve_cancel(ep_ctx* ctx, uint64_t ticket);
+int ep_serve_stop(ep_ctx* ctx);
+
+/* ---- host helpers ---- */
+/* Greedy legalization, mod.rs:206-256: cells in index order take the free
+ * BEL of their bucket nearest in squared distance, first minimum in BEL
+ * order.  bel_taken marks BELs already bound on entry and is updated. */
+int ep_legalize(int32_t n_cells, const double* x, const double* y, const int32_t* cell_bucket,
+                int32_t n_bels, const int32_t* bel_x, const int32_t* bel_y,
+                const int32_t* bel_bucket, uint8_t* bel_taken, int32_t* cell_bel,
+                double* displacement);
+
+/* Synthetic generic-FPGA design generator (clustered-locality netlists). */
+typedef struct ep_gen_params {
+    uint64_t seed;
+    int32_t n_cells;        /* movable */
+    int32_t n_fixed;        /* IO cells spaced evenly on the perimeter */
+    int32_t n_nets;         /* nets with the 60/25/12/3 % degree mix */
+    int32_t W, H, Bx, By;
+    int32_t n_clusters;
+    int32_t area_mix;       /* 0: unit cells; 1: areas 1/2/4 at 85/12/3 % */
+    int32_t n_types;        /* movable cell types (BEL buckets), 1 or 2 */
+    int32_t n_high_fanout;  /* extra nets of hf_min..hf_max pins */
+    int32_t hf_min, hf_max;
+    double locality;        /* fraction of a net's pins from its cluster (0.8) */
+    double type1_frac;      /* fraction of cells of type 1 (0.2 => 1200 LUT + 300 DFF) */
+} ep_gen_params;
+
+typedef struct ep_gen_design {
+    ep_design d;            /* arrays below, owned by the generator */
+    int32_t n_types;
+    int32_t n_bels;         /* logic BELs: one per type per tile, tile-major */
+    int32_t* cell_type;     /* [n_mov] */
+    int32_t* bel_x;         /* [n_bels] */
+    int32_t* bel_y;
+    int32_t* bel_type;
+    int32_t* init_bel;      /* [n_mov] BEL of the seeded initial placement */
+} ep_gen_design;
+
+int ep_generate(const ep_gen_params* p, ep_gen_design** out);
+void ep_gen_free(ep_gen_design* g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,256 @@
+"""ctypes binding of oracle/liboracle.so -- the FP64 CPU restatement of the
+reference placer.  Test infrastructure: used only as the checker."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_DIR = ROOT / "oracle"
+_p = C.c_void_p
+
+
+class OrcDesign(C.Structure):
+    _fields_ = [("n_mov", C.c_int32), ("n_fixed", C.c_int32), ("n_nets", C.c_int32),
+                ("W", C.c_int32), ("H", C.c_int32), ("Bx", C.c_int32), ("By", C.c_int32),
+                ("n_pins", C.c_int64), ("x0", _p), ("y0", _p), ("area", _p), ("fixed_x", _p),
+                ("fixed_y", _p), ("net_off", _p), ("pin_cell", _p), ("pinw", _p)]
+
+
+class OrcCfg(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("wl_coeff", C.c_double), ("target_density", C.c_double),
+                ("nesterov_step_size", C.c_double), ("max_iters", C.c_int32),
+                ("legalize_interval", C.c_int32), ("stop_every_iter", C.c_int32),
+                ("min_iters", C.c_int32), ("overflow_target", C.c_double),
+                ("field_mode", C.c_int32)]
+
+
+class OrcRec(C.Structure):
+    _fields_ = [("iter", C.c_int32), ("checked", C.c_int32), ("overflow", C.c_double),
+                ("lambda_", C.c_double), ("step_size_x", C.c_double), ("step_size_y", C.c_double),
+                ("step_norm_x", C.c_double), ("step_norm_y", C.c_double), ("hpwl", C.c_double),
+                ("wl_norm", C.c_double), ("den_norm", C.c_double)]
+
+
+class OrcResult(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("overflow", C.c_double),
+                ("hpwl", C.c_double), ("lambda_", C.c_double)]
+
+
+class OrcNesterov(C.Structure):
+    _fields_ = [("n", C.c_int), ("x", _p), ("x_prev", _p), ("step_size", C.c_double),
+                ("iter", C.c_int), ("a", C.c_double)]
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        so = ORACLE_DIR / "liboracle.so"
+        if not so.exists():
+            subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+        L = C.CDLL(str(so))
+        d = C.c_double
+        sig = {
+            "orc_wa_axis_value": (d, [_p, C.c_int, d]),
+            "orc_wa_axis_grad": (None, [_p, C.c_int, d, _p]),
+            "orc_wa_wirelength": (d, [_p, _p, C.c_int, d]),
+            "orc_wa_gradient": (None, [C.POINTER(OrcDesign), _p, _p, d, _p, _p]),
+            "orc_pin_weights": (None, [C.POINTER(OrcDesign), _p]),
+            "orc_bin_range": (None, [C.POINTER(OrcDesign), d, d, d, _p, _p]),
+            "orc_density": (None, [C.POINTER(OrcDesign), _p, _p, _p]),
+            "orc_overlap": (d, [_p, C.c_int64]),
+            "orc_dct2": (None, [_p, C.c_int]), "orc_dct3": (None, [_p, C.c_int]),
+            "orc_dst3": (None, [_p, C.c_int]),
+            "orc_dct2_naive": (None, [_p, _p, C.c_int]), "orc_dct3_naive": (None, [_p, _p, C.c_int]),
+            "orc_dst3_naive": (None, [_p, _p, C.c_int]),
+            "orc_field": (None, [_p, C.c_int, C.c_int, d, _p, _p]),
+            "orc_field_mode": (None, [_p, C.c_int, C.c_int, d, C.c_int, _p, _p]),
+            "orc_density_gradient": (None, [C.POINTER(OrcDesign), _p, _p, _p, _p, _p, _p]),
+            "orc_nesterov_init": (None, [C.POINTER(OrcNesterov), C.c_int, d, _p, _p]),
+            "orc_nesterov_set_positions": (None, [C.POINTER(OrcNesterov), _p]),
+            "orc_nesterov_momentum": (d, [C.POINTER(OrcNesterov)]),
+            "orc_nesterov_look_ahead": (None, [C.POINTER(OrcNesterov), _p]),
+            "orc_nesterov_step": (d, [C.POINTER(OrcNesterov), _p]),
+            "orc_nesterov_clamp": (None, [C.POINTER(OrcNesterov), d, d]),
+            "orc_nesterov_bb": (C.c_int, [C.POINTER(OrcNesterov), _p, _p, C.POINTER(d)]),
+            "orc_nesterov_lipschitz": (d, [C.POINTER(OrcNesterov), _p, _p]),
+            "orc_nesterov_adaptive_restart": (None, [C.POINTER(OrcNesterov), _p]),
+            "orc_hpwl": (d, [C.POINTER(OrcDesign), _p, _p]),
+            "orc_place": (C.c_int, [C.POINTER(OrcDesign), C.POINTER(OrcCfg), _p, _p,
+                                    C.POINTER(OrcRec), C.c_int, C.POINTER(C.c_int),
+                                    C.POINTER(OrcResult)]),
+            "orc_rng_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
+            "orc_rng_shuffle_u32": (None, [C.POINTER(C.c_uint64), _p, C.c_int]),
+            "orc_legalize": (C.c_int, [C.c_int, _p, _p, _p, C.c_int, _p, _p, _p, _p, _p, _p]),
+            "orc_total_hpwl_int": (d, [C.POINTER(OrcDesign), _p, _p]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _LIB = L
+    return _LIB
+
+
+def P(a):
+    return None if a is None else a.ctypes.data_as(_p)
+
+
+class Design:
+    """Keeps the numpy arrays alive behind an OrcDesign."""
+
+    def __init__(self, fd, x=None, y=None):
+        self.fd = fd
+        self.x0 = np.ascontiguousarray(fd.x0 if x is None else x, np.float64)
+        self.y0 = np.ascontiguousarray(fd.y0 if y is None else y, np.float64)
+        self.c = OrcDesign(fd.n_mov, fd.n_fixed, fd.n_nets, fd.W, fd.H, fd.Bx, fd.By, fd.n_pins,
+                           P(self.x0), P(self.y0), P(fd.area), P(fd.fixed_x), P(fd.fixed_y),
+                           P(fd.net_off), P(fd.pin_cell), P(fd.pinw))
+
+    @property
+    def ref(self):
+        return C.byref(self.c)
+
+
+def wa_gradient(fd, x, y, a=0.5):
+    D = Design(fd)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    gx, gy = np.zeros(fd.n_mov), np.zeros(fd.n_mov)
+    lib().orc_wa_gradient(D.ref, P(x), P(y), a, P(gx), P(gy))
+    return gx, gy
+
+
+def density(fd, x, y):
+    D = Design(fd)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    d = np.zeros(fd.Bx * fd.By)
+    lib().orc_density(D.ref, P(x), P(y), P(d))
+    return d.reshape(fd.By, fd.Bx)
+
+
+def overlap(dens):
+    dens = np.ascontiguousarray(dens, np.float64).reshape(-1)
+    return lib().orc_overlap(P(dens), dens.size)
+
+
+def bin_ranges(fd, x, y):
+    D = Design(fd)
+    out = np.zeros((len(x), 4), np.int32)
+    box = np.zeros(4)
+    r = np.zeros(4, np.int32)
+    for i in range(len(x)):
+        a = 1.0 if fd.area is None else float(fd.area[i])
+        lib().orc_bin_range(D.ref, float(x[i]), float(y[i]), a, P(r), P(box))
+        out[i] = r
+    return out
+
+
+def field(dens, Bx, By, mode=0, target_density=1.0):
+    dens = np.ascontiguousarray(dens, np.float64).reshape(-1)
+    ex, ey = np.zeros(Bx * By), np.zeros(Bx * By)
+    lib().orc_field_mode(P(dens), Bx, By, target_density, mode, P(ex), P(ey))
+    return ex.reshape(By, Bx), ey.reshape(By, Bx)
+
+
+def density_gradient(fd, x, y, ex, ey):
+    D = Design(fd)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    ex = np.ascontiguousarray(ex, np.float64).reshape(-1)
+    ey = np.ascontiguousarray(ey, np.float64).reshape(-1)
+    gx, gy = np.zeros(fd.n_mov), np.zeros(fd.n_mov)
+    lib().orc_density_gradient(D.ref, P(x), P(y), P(ex), P(ey), P(gx), P(gy))
+    return gx, gy
+
+
+def hpwl(fd, x, y):
+    D = Design(fd)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    return lib().orc_hpwl(D.ref, P(x), P(y))
+
+
+def place(fd, *, max_iters=300, wl_coeff=0.5, step=0.1, interval=5, min_iters=20,
+          overflow_target=0.1, field_mode=0, stop_every_iter=0, x0=None, y0=None):
+    D = Design(fd, x0, y0)
+    cfg = OrcCfg(1, wl_coeff, 1.0, step, max_iters, interval, stop_every_iter, min_iters,
+                 overflow_target, field_mode)
+    cap = max(1, max_iters)
+    recs = (OrcRec * cap)()
+    n = C.c_int()
+    res = OrcResult()
+    x = np.zeros(fd.n_mov)
+    y = np.zeros(fd.n_mov)
+    lib().orc_place(D.ref, C.byref(cfg), P(x), P(y), recs, cap, C.byref(n), C.byref(res))
+    keys = [f[0] for f in OrcRec._fields_]
+    log = [{k.rstrip("_"): getattr(recs[i], k) for k in keys} for i in range(n.value)]
+    return dict(iters=res.iters, converged=bool(res.converged), overflow=res.overflow,
+                hpwl=res.hpwl, lam=res.lambda_, x=x, y=y, log=log)
+
+
+def legalize(x, y, cell_bucket, bel_x, bel_y, bel_bucket, bel_taken=None):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    cb = np.ascontiguousarray(cell_bucket, np.int32)
+    bx = np.ascontiguousarray(bel_x, np.int32)
+    by = np.ascontiguousarray(bel_y, np.int32)
+    bb = np.ascontiguousarray(bel_bucket, np.int32)
+    taken = (np.zeros(bx.size, np.uint8) if bel_taken is None
+             else np.ascontiguousarray(bel_taken, np.uint8).copy())
+    out = np.full(x.size, -1, np.int32)
+    disp = C.c_double()
+    rc = lib().orc_legalize(x.size, P(x), P(y), P(cb), bx.size, P(bx), P(by), P(bb), P(taken),
+                            P(out), C.byref(disp))
+    return rc, out, disp.value
+
+
+class Nesterov:
+    def __init__(self, n, step):
+        self.x = np.zeros(max(n, 1))
+        self.xp = np.zeros(max(n, 1))
+        self.s = OrcNesterov()
+        lib().orc_nesterov_init(C.byref(self.s), n, step, P(self.x), P(self.xp))
+
+    def set_positions(self, v):
+        v = np.ascontiguousarray(v, np.float64)
+        lib().orc_nesterov_set_positions(C.byref(self.s), P(v))
+
+    def look_ahead(self):
+        y = np.zeros(self.s.n)
+        lib().orc_nesterov_look_ahead(C.byref(self.s), P(y))
+        return y
+
+    def step(self, g):
+        g = np.ascontiguousarray(g, np.float64)
+        return lib().orc_nesterov_step(C.byref(self.s), P(g))
+
+    def bb(self, gp, g):
+        gp = np.ascontiguousarray(gp, np.float64)
+        g = np.ascontiguousarray(g, np.float64)
+        out = C.c_double()
+        ok = lib().orc_nesterov_bb(C.byref(self.s), P(gp), P(g), C.byref(out))
+        return out.value if ok else None
+
+    def lipschitz(self, gp, g):
+        gp = np.ascontiguousarray(gp, np.float64)
+        g = np.ascontiguousarray(g, np.float64)
+        return lib().orc_nesterov_lipschitz(C.byref(self.s), P(gp), P(g))
+
+    def adaptive_restart(self, g):
+        g = np.ascontiguousarray(g, np.float64)
+        lib().orc_nesterov_adaptive_restart(C.byref(self.s), P(g))
+
+    def momentum(self):
+        return lib().orc_nesterov_momentum(C.byref(self.s))
+
+    @property
+    def positions(self):
+        return self.x[: self.s.n].copy()
