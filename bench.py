@@ -1,0 +1,264 @@
+"""Benchmark of the batched electrostatic placer on one MI355X.
+
+Workload (the batched configuration): 64 independent synthetic designs with
+log-uniform sizes of 2,000-100,000 movable cells (seed 0; design i uses seed
+i), nets = 1.1 x cells, grid side = round(sqrt(cells / 0.7)), bins = nearest
+power of two to the side in [64, 512], cell areas 1/2/4.  One step places the
+whole batch from its initial positions until every design reaches overflow
+<= 0.10 (checked every 5 iterations after iteration 20, as the reference
+does), all designs advancing together in one hipGraph-replayed launch
+sequence per iteration; FP32 bulk arithmetic, BF16 hi+lo MFMA transforms,
+FP64 reductions.  The spectral field is the index-shifted (IDXST) one: the
+reference's own field never reaches overflow 0.10 on these designs.
+
+value = designs placed per second over the timed steps (inputs resident in
+HBM before timing).  Also reported: p50/p95/p99 latency of a design within a
+step, the plain FP64 placer's designs/sec on the same designs (one design at
+a time), the HPWL deviation from it, the roofline of the dominant kernel
+(HIP-event timed live) and the CPU oracle on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--designs", type=int, default=64)
+    p.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU oracle sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--profile-only", action="store_true", help="one step, no baselines (for rocprof)")
+    p.add_argument("--serve", type=float, default=0.0,
+                   help="serving scenario: Poisson arrival rate in requests/s (0: batch workload)")
+    p.add_argument("--duration", type=float, default=120.0, help="serving: arrival window in seconds")
+    p.add_argument("--pool", type=int, default=96, help="serving: distinct pre-generated designs")
+    return p.parse_args()
+
+
+def serve(args):
+    """Serving scenario: requests arrive as a Poisson process (seed 1) for
+    `duration` seconds, sizes log-uniform 1,000-500,000 cells (batch rule for
+    grid and bins), admitted against a 200 GB budget with up to 128 designs per
+    fused iteration.  Requests are drawn from a pool of pre-generated designs
+    so generation stays off the clock."""
+    import eisenjoch_amd as ep
+    rng = np.random.default_rng(1)
+    sizes = [int(round(math.exp(v))) for v in rng.uniform(math.log(1000), math.log(500_000), args.pool)]
+    pool = [ep.sized_design(n, 1000 + i) for i, n in enumerate(sizes)]
+    arrivals = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / args.serve)
+        if t > args.duration:
+            break
+        arrivals.append((t, int(rng.integers(0, len(pool)))))
+    cfg = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
+    dev = ep.Device(0, hbm_budget_bytes=200 << 30)
+    srv = ep.Server(dev, max_slots=128, iters_per_sync=5)
+    # warm the transform cache and code paths outside the clock
+    w = srv.submit(pool[0], cfg)
+    while srv.poll(w) is None:
+        time.sleep(0.001)
+    srv.release(w)
+    tickets = []
+    t0 = time.perf_counter()
+    k = 0
+    last_print = t0
+    while k < len(arrivals) or any(tk[2] is None for tk in tickets):
+        now = time.perf_counter() - t0
+        while k < len(arrivals) and arrivals[k][0] <= now:
+            tickets.append([srv.submit(pool[arrivals[k][1]], cfg), arrivals[k][1], None])
+            k += 1
+        for tk in tickets:
+            if tk[2] is None:
+                r = srv.poll(tk[0])
+                if r is not None:
+                    tk[2] = r
+                    srv.release(tk[0])
+        if time.perf_counter() - last_print > 10:
+            last_print = time.perf_counter()
+            print(f"[serve] t={now:.0f}s submitted {k}/{len(arrivals)} "
+                  f"done {sum(tk[2] is not None for tk in tickets)}", file=sys.stderr, flush=True)
+        time.sleep(0.0005)
+    wall = time.perf_counter() - t0
+    srv.stop()
+    res = [tk[2] for tk in tickets]
+    lat = np.array([r.t_done_ms - r.t_submit_ms for r in res])
+    ok = [r for r in res if r.status == 0 and r.converged]
+    out = {
+        "metric": "serving: designs/sec and p50/p95/p99 time to overflow<=0.10",
+        "value": round(len(ok) / wall, 3), "unit": "designs/s", "n_gpus": 1,
+        "higher_is_better": True, "rate_req_s": args.serve, "duration_s": args.duration,
+        "requests": len(res), "converged": len(ok), "wall_s": round(wall, 2),
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p95": float(np.percentile(lat, 95)),
+                       "p99": float(np.percentile(lat, 99)), "max": float(lat.max())},
+        "config": {"workload": "serving: Poisson arrivals, log-uniform 1k-500k cells, 128 slots, 200 GB",
+                   "pool": len(pool), "seed": 1},
+        "dtype": "f32+bf16x3-mfma+f64-reductions", "data": "synthetic (seeded generator)",
+    }
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+def algorithmic_work(designs):
+    """Per-iteration algorithmic bytes/flops of each kernel, summed over the
+    batch (FP32 bulk, 4-byte indices)."""
+    wa = scat = gath = 0.0
+    dct = 0.0
+    for d in designs:
+        N, F, E, P = d.n_mov, d.n_fixed, d.n_nets, d.n_pins
+        B2 = d.Bx * d.By
+        wa += 4 * (E + 1) + 4 * P + 8 * N + 8 * F + 8 * N
+        scat += 8 * N + 4 * N + 4 * B2
+        gath += 60 * N + 8 * B2
+        # six products of 2 M N K flops (three with K = Bx, three with K = By),
+        # three BF16 MFMA products per FP32 one: 18 Bx By (Bx + By) = 36 B^3
+        dct += 3 * 2 * d.Bx * d.By * (3 * d.Bx + 3 * d.By)
+    return {"wa": wa, "scatter": scat, "gather+update": gath, "dct_flops": dct}
+
+
+def cpu_baseline(designs, budget_s):
+    """FP64 oracle (tests/_oracle.py, the checker) on a bounded sample of the
+    batch, one core; returns designs/sec over the designs it finished."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as O
+    order = list(range(0, len(designs), 8)) + [i for i in range(len(designs)) if i % 8]
+    t0 = time.perf_counter()
+    done = []
+    for i in order:
+        if time.perf_counter() - t0 > budget_s:
+            break
+        O.place(designs[i], max_iters=500, field_mode=1)
+        done.append(i)
+    dt = time.perf_counter() - t0
+    return {"value": round(len(done) / dt, 4), "unit": "designs/s", "cores": 1, "kind": "port",
+            "sample": f"{len(done)} of the {len(designs)} batch designs (indices {done}, "
+                      f"{sum(designs[i].n_mov for i in done)} cells) placed to overflow <= 0.10 "
+                      f"by the FP64 C oracle in {dt:.1f} s on one host core"}
+
+
+def main():
+    args = parse()
+    if args.serve > 0 and args.gpus == 1:
+        serve(args)
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this placer runs on one GPU; designs shard across GPUs "
+                          "without communication, so run one process per GPU"}))
+        sys.exit(2)
+    import eisenjoch_amd as ep
+
+    sizes = ep.batch_sizes(args.designs)
+    designs = [ep.sized_design(n, i) for i, n in enumerate(sizes)]
+    cfg = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
+    dev = ep.Device(0)
+    ids = [dev.upload(d, "fast") for d in designs]
+
+    for _ in range(args.warmup):
+        dev.place_batch(ids, cfg, positions=False)
+    steps = 1 if args.profile_only else args.steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = dev.place_batch(ids, cfg, positions=False)
+    dt = time.perf_counter() - t0
+    value = len(ids) * steps / dt
+    lat = np.array([r.t_done_ms for r in res])
+    conv = sum(r.converged for r in res)
+    iters = [r.iters for r in res]
+    out = {
+        "metric": "designs/sec to overflow<=0.10 (batched, 64 designs), p99 latency, HPWL vs FP64 plain",
+        "value": round(value, 3),
+        "unit": "designs/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+bf16x3-mfma+f64-reductions",
+        "data": "synthetic (seeded generator)",
+        "config": {
+            "workload": "batched: 64 designs, log-uniform 2k-100k cells, nets 1.1x, side sqrt(N/0.7), "
+                        "pow2 bins 64-512, areas 1/2/4, idxst field, stop overflow<=0.10",
+            "designs": len(ids),
+            "total_cells": int(sum(d.n_mov for d in designs)),
+            "total_pins": int(sum(d.n_pins for d in designs)),
+            "converged": int(conv),
+            "iters_min_max": [int(min(iters)), int(max(iters))],
+        },
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p95": float(np.percentile(lat, 95)),
+                       "p99": float(np.percentile(lat, 99))},
+    }
+    if args.profile_only:
+        print(json.dumps(out), flush=True)
+        return
+
+    # plain FP64 GPU placer, one design at a time (reference point, not timed above)
+    if not args.no_plain:
+        plain_ids = [dev.upload(d, "fp64") for d in designs]
+        cfg64 = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
+        dev.place(plain_ids[0], cfg64, positions=False)  # build transforms outside the timing
+        t0 = time.perf_counter()
+        plain = [dev.place(i, cfg64, positions=False) for i in plain_ids]
+        pdt = time.perf_counter() - t0
+        dev_h = [abs(a.hpwl - b.hpwl) / b.hpwl for a, b in zip(res, plain)]
+        out["plain_fp64"] = {"designs_per_s": round(len(plain) / pdt, 3),
+                             "speedup_batched_vs_plain": round(value / (len(plain) / pdt), 2),
+                             "hpwl_rel_dev_max": float(max(dev_h)),
+                             "hpwl_rel_dev_mean": float(np.mean(dev_h)),
+                             "plain_converged": int(sum(p.converged for p in plain))}
+        for i in plain_ids:
+            dev.free(i)
+
+    # roofline of the dominant kernel, HIP events on the context stream
+    prof = dev.profile(ids, cfg, warm=8, reps=5)
+    work = algorithmic_work(designs)
+    dct_ms = sum(prof[k] for k in ("dct_f1", "dct_f2", "dct_i1", "dct_i2"))
+    hbm = {"wa": (prof["wa"], work["wa"]), "scatter": (prof["scatter"], work["scatter"]),
+           "gather+update": (prof["gather"] + prof["update"], work["gather+update"])}
+    cands = {k: v[0] for k, v in hbm.items()}
+    cands["dct"] = dct_ms
+    dom = max(cands, key=cands.get)
+    if dom == "dct":
+        ach = work["dct_flops"] / (dct_ms * 1e-3) / 1e12
+        out["roofline"] = {"kernel": "dct (4 transform passes, BF16 hi+lo MFMA)", "bound": "mfma",
+                           "achieved": round(ach, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(ach / BF16_PEAK_TFLOPS, 5), "traffic": None}
+    else:
+        ms, b = hbm[dom]
+        ach = b / (ms * 1e-3) / 1e9
+        out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                           "traffic": None}
+    out["phase_ms_per_iter"] = {k: round(v, 4) for k, v in prof.items()}
+    out["kernel_rooflines"] = {
+        k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in hbm.items() if v[0] > 0}
+    out["kernel_rooflines"]["dct_tflops"] = round(work["dct_flops"] / (dct_ms * 1e-3) / 1e12, 2) if dct_ms else None
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(designs, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

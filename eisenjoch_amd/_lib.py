@@ -81,6 +81,11 @@ class EpServeConfig(C.Structure):
                 ("hbm_budget_bytes", C.c_uint64)]
 
 
+class EpPhaseTime(C.Structure):
+    _fields_ = [("name", C.c_char * 24), ("launches", C.c_int32), ("reserved", C.c_int32),
+                ("avg_ms", C.c_double)]
+
+
 class EpGenParams(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64), ("n_cells", C.c_int32), ("n_fixed", C.c_int32),
@@ -129,6 +134,9 @@ SIGNATURES = {
     "ep_serve_release": (C.c_int, [_p, C.c_uint64]),
     "ep_serve_cancel": (C.c_int, [_p, C.c_uint64]),
     "ep_serve_stop": (C.c_int, [_p]),
+    "ep_profile_iteration": (C.c_int, [_p, C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(EpConfig),
+                                       C.c_uint32, C.c_uint32, C.POINTER(EpPhaseTime), C.c_uint32,
+                                       C.POINTER(C.c_uint32)]),
     "ep_legalize": (C.c_int, [C.c_int32, _p, _p, _p, C.c_int32, _p, _p, _p, _p, _p, _p]),
     "ep_generate": (C.c_int, [C.POINTER(EpGenParams), C.POINTER(C.POINTER(EpGenDesign))]),
     "ep_gen_free": (None, [C.POINTER(EpGenDesign)]),

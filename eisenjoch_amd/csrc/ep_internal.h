@@ -20,6 +20,8 @@ constexpr uint32_t kPad = 0xFFFFFFFFu;  // empty pin slot
 constexpr int kCellParts = 8;        // partial sums per cell block
 constexpr int kBinParts = 2;         // partial sums per bin block
 constexpr double kFixScale = 4294967296.0;  // 2^32 fixed point for deterministic sums
+constexpr int kScatCells = 1024;     // cells per scatter block
+constexpr int kScatBins = 6144;      // LDS accumulation window of a scatter block (bins)
 
 // Field modes (the transform constants differ, nothing else).
 enum FieldMode : int32_t { kFieldReference = 0, kFieldIdxst = 1 };
@@ -59,7 +61,8 @@ struct DevCfg {
 // (T = double for EP_PREC_FP64, float for EP_PREC_FAST).
 struct DesignDev {
     int32_t n_mov, n_fixed, n_nets, W, H, Bx, By, prec;
-    int32_t n_cell_blocks, n_bin_blocks, n_wa_waves, n_hf, n_buckets, pad1;
+    int32_t n_cell_blocks, n_bin_blocks, n_wa_waves, n_hf, n_buckets, n_scat_blocks;
+    int32_t n_slots, pad1;        // per-pin gradient slots: n_wa_waves * 64 + high-degree pins
     int32_t bucket_wave_start[kMaxBuckets + 1];
     int32_t bucket_group[kMaxBuckets];
     double rx, ry, max_x, max_y;
@@ -72,8 +75,9 @@ struct DesignDev {
     void* dpos[2];             // x - x_prev
     void* look[2];             // clamped look-ahead
     void* g[2][2];             // [parity][axis] combined gradient (or density grad)
-    void* gwl[2];              // FP64: double WA gradient accumulators
-    long long* acc[2];         // FAST: fixed-point WA gradient accumulators
+    void* pg;                  // per-slot WA gradient (gx, gy) in T [n_slots][2]
+    uint32_t* cs_off;          // cell -> slot CSR [n_mov + 1]
+    uint32_t* cs_idx;          // slots of each movable cell, ascending
     void* pinw;                // T[n_mov]
     float* hs;                 // half side of the footprint [n_mov]
     void* fixed[2];            // T[n_fixed]
@@ -127,12 +131,13 @@ struct BatchDev {
     const int32_t* pfx_bin;
     const int32_t* pfx_wa;     // blocks of 4 waves
     const int32_t* pfx_hf;     // one block per high-degree net
+    const int32_t* pfx_scat;   // scatter blocks (kScatCells cells each)
 };
 
 // ---- launchers (ep_kernels.hip) ----
 void launch_wa(const BatchDev& b, int prec, int total_wa_blocks, int total_hf_blocks,
                hipStream_t s);
-void launch_scatter(const BatchDev& b, int prec, int total_cell_blocks, hipStream_t s);
+void launch_scatter(const BatchDev& b, int prec, int total_scat_blocks, hipStream_t s);
 void launch_dens_final(const BatchDev& b, int prec, int total_bin_blocks, hipStream_t s);
 void launch_gather(const BatchDev& b, int prec, int total_cell_blocks, int mode, hipStream_t s);
 void launch_ctrl(const BatchDev& b, int which, hipStream_t s);
@@ -143,6 +148,8 @@ void launch_reset(const BatchDev& b, int prec, int total_cell_blocks, hipStream_
 void launch_bin_ranges(const BatchDev& b, int total_cell_blocks, int32_t* out, hipStream_t s);
 void launch_gather_export(const BatchDev& b, int prec, int total_cell_blocks, double* gx,
                           double* gy, hipStream_t s);
+void launch_wa_reduce(const BatchDev& b, int prec, int total_cell_blocks, double* gx, double* gy,
+                      hipStream_t s);
 
 // ---- transforms (ep_gemm.hip) ----
 // Runs one pass (all jobs) of the spectral solve; engine 0 = MFMA, 1 = FMA.

@@ -30,6 +30,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 #include "ep_internal.h"
 
@@ -102,17 +103,14 @@ __device__ inline void pin_pos(const DesignDev& D, uint32_t c, const T* lx, cons
     }
 }
 
-template <typename T, bool FIX>
-__device__ inline void acc_grad(const DesignDev& D, uint32_t c, T gx, T gy) {
-    if constexpr (FIX) {
-        atomicAdd((unsigned long long*)&D.acc[0][c],
-                  (unsigned long long)__double2ll_rn((double)gx * kFixScale));
-        atomicAdd((unsigned long long*)&D.acc[1][c],
-                  (unsigned long long)__double2ll_rn((double)gy * kFixScale));
-    } else {
-        atomicAdd(&((double*)D.gwl[0])[c], (double)gx);
-        atomicAdd(&((double*)D.gwl[1])[c], (double)gy);
-    }
+// Per-pin gradients go to their own slot; each cell later sums its slots in
+// a fixed order (cell -> slot CSR), so there are no atomics and the sum is
+// deterministic.
+template <typename T>
+__device__ inline void store_grad(const DesignDev& D, uint32_t slot, T gx, T gy) {
+    T* pg = (T*)D.pg;
+    pg[2 * (size_t)slot] = gx;
+    pg[2 * (size_t)slot + 1] = gy;
 }
 
 // HPWL mode: per-wave partial of sum over nets of (max - min) x + y of the
@@ -149,7 +147,10 @@ __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
         ymn = fmin(ymn, shfl_xor(ymn, o));
     }
     if (HPWL) {
-        double h = ((lane & (G - 1)) == 0) ? (double)(xmx - xmn) + (double)(ymx - ymn) : 0.0;
+        // wave padding forms whole groups without a valid pin (max < min): skip them
+        const bool has = xmx >= xmn;
+        double h = ((lane & (G - 1)) == 0 && has)
+                       ? ((double)xmx - (double)xmn) + ((double)ymx - (double)ymn) : 0.0;
         for (int o = 32; o >= 1; o >>= 1) h += __shfl_xor(h, o, kWave);
         if (lane == 0) D.part_net[w] = h;
         return;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
     if (valid && c < (uint32_t)D.n_mov) {
         T gx = wa_pin<T>(a, ux, ex, s[0], s[1], vx, fx, s[2], s[3]);
         T gy = wa_pin<T>(a, uy, ey, s[4], s[5], vy, fy, s[6], s[7]);
-        acc_grad<T, FIX>(D, c, gx, gy);
+        store_grad<T>(D, (uint32_t)w * kWave + lane, gx, gy);
     }
 }
 
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(256) void k_wa_hf(BatchDev b, int final_pass) {
     const T xmx = (T)mm[0], xmn = (T)(-mm[1]), ymx = (T)mm[2], ymn = (T)(-mm[3]);
     if (HPWL) {
         if (threadIdx.x == 0)
-            D.part_net[D.n_wa_waves + e] = (double)(xmx - xmn) + (double)(ymx - ymn);
+            D.part_net[D.n_wa_waves + e] = ((double)xmx - (double)xmn) + ((double)ymx - (double)ymn);
         return;
     }
     const T a = (T)D.cfg.wl_coeff;
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(256) void k_wa_hf(BatchDev b, int final_pass) {
         T ex = fexp(a * ux), fx = fexp(-a * vx), ey = fexp(a * uy), fy = fexp(-a * vy);
         T gx = wa_pin<T>(a, ux, ex, S[0], S[1], vx, fx, S[2], S[3]);
         T gy = wa_pin<T>(a, uy, ey, S[4], S[5], vy, fy, S[6], S[7]);
-        acc_grad<T, FIX>(D, c, gx, gy);
+        store_grad<T>(D, (uint32_t)D.n_wa_waves * kWave + p, gx, gy);
     }
 }
 
@@ -285,25 +286,68 @@ __device__ inline double ov_area(const BinBox& r, int bx, int by) {
     return ox * oy;
 }
 
+// One block covers kScatCells consecutive cells.  Cells of a cluster have
+// neighbouring indices and sit close together once placement has started,
+// so when the block's bin bounding box fits kScatBins the contributions are
+// summed in LDS and flushed with one global atomic per touched bin;
+// otherwise each contribution goes to global memory directly.  All sums are
+// fixed point (FAST) so the result does not depend on arrival order.
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
-    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    typedef typename std::conditional<FIX, unsigned long long, double>::type Acc;
+    __shared__ Acc win[kScatBins];
+    __shared__ int box[4];
+    const int d = find_blk(b.pfx_scat, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
-    const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
-    if (i >= D.n_mov) return;
-    const double cx = (double)((const T*)D.look[0])[i], cy = (double)((const T*)D.look[1])[i];
-    const BinBox r = bin_box(D, cx, cy, (double)D.hs[i]);
-    for (int by = r.by0; by < r.by1; by++)
-        for (int bx = r.bx0; bx < r.bx1; bx++) {
-            double w = ov_area(r, bx, by);
-            size_t idx = (size_t)by * D.Bx + bx;
-            if constexpr (FIX)
-                atomicAdd(&((unsigned long long*)D.dacc)[idx],
-                          (unsigned long long)__double2ll_rn(w * kFixScale));
-            else
-                atomicAdd(&((double*)D.dacc)[idx], w);
+    const int c0 = (blockIdx.x - b.pfx_scat[d]) * kScatCells;
+    const T* LX = (const T*)D.look[0];
+    const T* LY = (const T*)D.look[1];
+    if (threadIdx.x == 0) {
+        box[0] = box[2] = 0x7fffffff;
+        box[1] = box[3] = -1;
+    }
+    __syncthreads();
+    int lo_x = 0x7fffffff, hi_x = -1, lo_y = 0x7fffffff, hi_y = -1;
+    for (int k = threadIdx.x; k < kScatCells; k += blockDim.x) {
+        const int i = c0 + k;
+        if (i >= D.n_mov) break;
+        const BinBox r = bin_box(D, (double)LX[i], (double)LY[i], (double)D.hs[i]);
+        if (r.bx1 > r.bx0 && r.by1 > r.by0) {
+            lo_x = min(lo_x, r.bx0); hi_x = max(hi_x, r.bx1);
+            lo_y = min(lo_y, r.by0); hi_y = max(hi_y, r.by1);
         }
+    }
+    atomicMin(&box[0], lo_x); atomicMax(&box[1], hi_x);
+    atomicMin(&box[2], lo_y); atomicMax(&box[3], hi_y);
+    __syncthreads();
+    const int bx0 = box[0], by0 = box[2];
+    const int ww = box[1] - box[0], hh = box[3] - box[2];
+    const bool local = ww > 0 && hh > 0 && (long long)ww * hh <= kScatBins;
+    if (local)
+        for (int k = threadIdx.x; k < ww * hh; k += blockDim.x) win[k] = Acc(0);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kScatCells; k += blockDim.x) {
+        const int i = c0 + k;
+        if (i >= D.n_mov) break;
+        const BinBox r = bin_box(D, (double)LX[i], (double)LY[i], (double)D.hs[i]);
+        for (int by = r.by0; by < r.by1; by++)
+            for (int bx = r.bx0; bx < r.bx1; bx++) {
+                const double w = ov_area(r, bx, by);
+                Acc v;
+                if constexpr (FIX) v = (Acc)__double2ll_rn(w * kFixScale);
+                else v = w;
+                if (local) atomicAdd(&win[(by - by0) * ww + (bx - bx0)], v);
+                else atomicAdd(&((Acc*)D.dacc)[(size_t)by * D.Bx + bx], v);
+            }
+    }
+    if (!local) return;
+    __syncthreads();
+    for (int k = threadIdx.x; k < ww * hh; k += blockDim.x) {
+        const Acc v = win[k];
+        if (v != Acc(0))
+            atomicAdd(&((Acc*)D.dacc)[(size_t)(by0 + k / ww) * D.Bx + bx0 + k % ww], v);
+    }
 }
 
 template <typename T, bool FIX>
@@ -338,10 +382,18 @@ __global__ __launch_bounds__(256) void k_dens_final(BatchDev b) {
 
 // ------------------------------------------------------------ gather ----
 
-template <typename T, bool FIX>
-__device__ inline double gwl_of(const DesignDev& D, int axis, int i) {
-    if constexpr (FIX) return (double)D.acc[axis][i] * (1.0 / kFixScale);
-    else return ((const double*)D.gwl[axis])[i];
+// WA gradient of cell i: its pin slots summed in ascending slot order.
+template <typename T>
+__device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy) {
+    const T* pg = (const T*)D.pg;
+    double sx = 0.0, sy = 0.0;
+    for (uint32_t k = D.cs_off[i], e = D.cs_off[i + 1]; k < e; k++) {
+        const size_t s = D.cs_idx[k];
+        sx += (double)pg[2 * s];
+        sy += (double)pg[2 * s + 1];
+    }
+    gx = sx;
+    gy = sy;
 }
 
 // MODE 0: gather; MODE 1: fix (penalty set by ctrl this iteration);
@@ -390,7 +442,8 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
             ey_out[i] = gden[1];
         } else if (MODE == 0 && !st->lam_init) {
             // penalty not yet initialised: park the density gradient, norms only
-            double w0 = gwl_of<T, FIX>(D, 0, i), w1 = gwl_of<T, FIX>(D, 1, i);
+            double w0, w1;
+            gwl_of<T>(D, i, w0, w1);
             ((T*)D.g[par][0])[i] = (T)gden[0];
             ((T*)D.g[par][1])[i] = (T)gden[1];
             part[0] = w0 * w0 + w1 * w1;
@@ -399,9 +452,11 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
             const double lam = MODE == 0 ? sched(D.cfg, st->lam) : st->lam;
             double pc = ((const T*)D.pinw)[i] + lam;  // mod.rs:152-156
             pc = pc < 1.0 ? 1.0 : pc;
+            double gwa[2];
+            gwl_of<T>(D, i, gwa[0], gwa[1]);
 #pragma unroll
             for (int ax = 0; ax < 2; ax++) {
-                double gw = gwl_of<T, FIX>(D, ax, i);
+                const double gw = gwa[ax];
                 T g = (T)((gw + lam * gden[ax]) / pc);  // mod.rs:146-149
                 ((T*)D.g[par][ax])[i] = g;
                 if (it > 0) {
@@ -598,8 +653,6 @@ __global__ __launch_bounds__(256) void k_update(BatchDev b) {
             T l = xnc + mn * ndx;  // nesterov.rs:95-101 then common.rs:410-415
             l = l < T(0) ? T(0) : (l > hi ? hi : l);
             ((T*)D.look[ax])[i] = l;
-            if constexpr (FIX) D.acc[ax][i] = 0;
-            else ((double*)D.gwl[ax])[i] = 0.0;
         }
     }
     block_sum<2>(sq, sh);
@@ -624,9 +677,20 @@ __global__ __launch_bounds__(256) void k_reset(BatchDev b) {
         ((T*)D.pos[ax])[i] = x;
         ((T*)D.dpos[ax])[i] = T(0);
         ((T*)D.look[ax])[i] = x < T(0) ? T(0) : (x > hi ? hi : x);
-        if constexpr (FIX) D.acc[ax][i] = 0;
-        else ((double*)D.gwl[ax])[i] = 0.0;
     }
+}
+
+// Evaluation helper: the WA gradient per cell (the gather's slot reduction).
+template <typename T>
+__global__ __launch_bounds__(256) void k_wa_reduce(BatchDev b, double* gx, double* gy) {
+    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    if (i >= D.n_mov) return;
+    double a, c;
+    gwl_of<T>(D, i, a, c);
+    gx[i] = a;
+    gy[i] = c;
 }
 
 // Evaluation helper: integer bin ranges per cell.
@@ -683,12 +747,20 @@ void launch_hpwl(const BatchDev& b, int prec, int wa_blocks, int hf_blocks, int 
     }
 }
 
-void launch_scatter(const BatchDev& b, int prec, int cell_blocks, hipStream_t s) {
+void launch_scatter(const BatchDev& b, int prec, int scat_blocks, hipStream_t s) {
+    if (scat_blocks <= 0) return;
+    if (prec == EP_PREC_FP64)
+        hipLaunchKernelGGL((k_scatter<double, false>), dim3(scat_blocks), dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL((k_scatter<float, true>), dim3(scat_blocks), dim3(256), 0, s, b);
+}
+
+void launch_wa_reduce(const BatchDev& b, int prec, int cell_blocks, double* gx, double* gy, hipStream_t s) {
     if (cell_blocks <= 0) return;
     if (prec == EP_PREC_FP64)
-        hipLaunchKernelGGL((k_scatter<double, false>), dim3(cell_blocks), dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_wa_reduce<double>, dim3(cell_blocks), dim3(256), 0, s, b, gx, gy);
     else
-        hipLaunchKernelGGL((k_scatter<float, true>), dim3(cell_blocks), dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_wa_reduce<float>, dim3(cell_blocks), dim3(256), 0, s, b, gx, gy);
 }
 
 void launch_dens_final(const BatchDev& b, int prec, int bin_blocks, hipStream_t s) {

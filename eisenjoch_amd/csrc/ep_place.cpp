@@ -65,8 +65,8 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         if (cfgs[i]->log_hpwl) b.log_hpwl = 1;
     }
     std::vector<DesignDev> dv(nd);
-    std::vector<int32_t> pc(nd + 1), pb(nd + 1), pw(nd + 1), ph(nd + 1);
-    pc[0] = pb[0] = pw[0] = ph[0] = 0;
+    std::vector<int32_t> pc(nd + 1), pb(nd + 1), pw(nd + 1), ph(nd + 1), ps(nd + 1);
+    pc[0] = pb[0] = pw[0] = ph[0] = ps[0] = 0;
     // jobs: [pass][engine]
     std::vector<GemmJob> jobs[4][2];
     for (int i = 0; i < nd; i++) {
@@ -77,6 +77,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         pb[i + 1] = pb[i] + D.dev.n_bin_blocks;
         pw[i + 1] = pw[i] + (D.dev.n_wa_waves + 3) / 4;
         ph[i + 1] = ph[i] + D.dev.n_hf;
+        ps[i + 1] = ps[i] + D.dev.n_scat_blocks;
         Transform *tx, *ty;
         int rc = get_transform(c, D.dev.Bx, cfgs[i]->field_mode, &tx);
         if (!rc) rc = get_transform(c, D.dev.By, cfgs[i]->field_mode, &ty);
@@ -125,8 +126,9 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     b.bin_blocks = pb[nd];
     b.wa_blocks = pw[nd];
     b.hf_blocks = ph[nd];
+    b.scat_blocks = ps[nd];
     // device image: descriptors, prefixes, jobs
-    size_t bytes = align_up(sizeof(DesignDev) * nd) + 4 * align_up(sizeof(int32_t) * (nd + 1)) + align_up(4);
+    size_t bytes = align_up(sizeof(DesignDev) * nd) + 5 * align_up(sizeof(int32_t) * (nd + 1)) + align_up(4);
     for (int p = 0; p < 4; p++)
         for (int e = 0; e < 2; e++) {
             int t0 = 0;
@@ -154,6 +156,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     b.bd.pfx_bin = (const int32_t*)put(pb.data(), 4 * (nd + 1));
     b.bd.pfx_wa = (const int32_t*)put(pw.data(), 4 * (nd + 1));
     b.bd.pfx_hf = (const int32_t*)put(ph.data(), 4 * (nd + 1));
+    b.bd.pfx_scat = (const int32_t*)put(ps.data(), 4 * (nd + 1));
     const int32_t zero = 0;
     b.dzero = (const int32_t*)put(&zero, 4);
     for (int p = 0; p < 4; p++)
@@ -233,19 +236,30 @@ static void enqueue_gemm(Ctx* c, Batch& b, int p) {
             launch_gemm_pass(b.jobs2[p][e], b.njobs2[p][e], b.tiles2[p][e], b.d_dev, b.prec, e, c->stream);
 }
 
+// Phases of one iteration, in launch order (names used by the profiler).
+static const char* kPhaseNames[] = {"wa", "scatter", "dens_final", "dct_f1", "dct_f2", "dct_i1",
+                                    "dct_i2", "gather", "ctrl_a", "fix", "ctrl_b", "update", "hpwl"};
+constexpr int kNumPhases = 13;
+
+static void enqueue_phase(Ctx* c, Batch& b, int ph) {
+    hipStream_t s = c->stream;
+    switch (ph) {
+        case 0: launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, s); break;  // mod.rs:110-115
+        case 1: launch_scatter(b.bd, b.prec, b.scat_blocks, s); break;        // mod.rs:118
+        case 2: launch_dens_final(b.bd, b.prec, b.bin_blocks, s); break;      // mod.rs:119
+        case 3: case 4: case 5: case 6: enqueue_gemm(c, b, ph - 3); break;   // mod.rs:121-123
+        case 7: launch_gather(b.bd, b.prec, b.cell_blocks, 0, s); break;      // mod.rs:126-156
+        case 8: launch_ctrl(b.bd, 0, s); break;                               // mod.rs:131-143, 159-166
+        case 9: launch_gather(b.bd, b.prec, b.cell_blocks, 1, s); break;
+        case 10: launch_ctrl(b.bd, 1, s); break;                              // nesterov.rs:41-47, mod.rs:176-195
+        case 11: launch_update(b.bd, b.prec, b.cell_blocks, s); break;        // mod.rs:169-173, 105-107
+        case 12: if (b.log_hpwl) launch_hpwl(b.bd, b.prec, b.wa_blocks, b.hf_blocks, 0, s); break;
+    }
+}
+
 // One iteration of mod.rs:104-197 for every live design of the batch.
 static void enqueue_iteration(Ctx* c, Batch& b) {
-    hipStream_t s = c->stream;
-    launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, s);     // mod.rs:110-115
-    launch_scatter(b.bd, b.prec, b.cell_blocks, s);           // mod.rs:118
-    launch_dens_final(b.bd, b.prec, b.bin_blocks, s);         // mod.rs:119
-    for (int p = 0; p < 4; p++) enqueue_gemm(c, b, p);        // mod.rs:121-123
-    launch_gather(b.bd, b.prec, b.cell_blocks, 0, s);         // mod.rs:126-156
-    launch_ctrl(b.bd, 0, s);                                  // mod.rs:131-143, 159-166
-    launch_gather(b.bd, b.prec, b.cell_blocks, 1, s);
-    launch_ctrl(b.bd, 1, s);                                  // nesterov.rs:41-47, mod.rs:176-195
-    launch_update(b.bd, b.prec, b.cell_blocks, s);            // mod.rs:169-173, 105-107
-    if (b.log_hpwl) launch_hpwl(b.bd, b.prec, b.wa_blocks, b.hf_blocks, 0, s);
+    for (int ph = 0; ph < kNumPhases; ph++) enqueue_phase(c, b, ph);
 }
 
 int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph) {
@@ -306,15 +320,22 @@ void fill_result(const Design& D, const DevState& s, ep_result* r) {
 
 // Runs a built batch until every design stops.
 int batch_place(Ctx* c, Batch& b, int sync_every, bool use_graph) {
+    const double t0 = now_ms();
     int rc = batch_reset(c, b, nullptr);
     if (rc) return rc;
     int max_it = 0;
     for (auto& cf : b.cfgs) max_it = std::max(max_it, cf.max_iters);
     int done_iters = 0;
+    b.done_ms.assign(b.ds.size(), -1.0);
     for (;;) {
         bool all = true;
         if ((rc = read_states(c, b))) return rc;
-        for (auto* D : b.ds) all = all && c->state_host[D->state_idx].done;
+        const double t = now_ms() - t0;
+        for (size_t k = 0; k < b.ds.size(); k++) {
+            const bool dn = c->state_host[b.ds[k]->state_idx].done;
+            if (dn && b.done_ms[k] < 0) b.done_ms[k] = t;
+            all = all && dn;
+        }
         if (all || done_iters >= max_it) break;
         if ((rc = batch_run_iters(c, b, sync_every, use_graph))) return rc;
         done_iters += sync_every;
@@ -405,6 +426,9 @@ int ep_place_batch(ep_ctx* h, const uint32_t* ids, uint32_t n, const ep_config* 
     int worst = EP_OK;
     for (uint32_t i = 0; i < n; i++) {
         fill_result(*ds[i], c->state_host[ds[i]->state_idx], &res[i]);
+        res[i].t_submit_ms = 0.0;
+        res[i].t_admit_ms = 0.0;
+        res[i].t_done_ms = i < b->done_ms.size() ? b->done_ms[i] : 0.0;
         if (res[i].status != EP_OK) worst = res[i].status;
     }
     if (worst == EP_ERR_NONFINITE) fail(c, worst, "non-finite value in the placement state");
@@ -454,6 +478,54 @@ int ep_iter_log(ep_ctx* h, uint32_t id, ep_iter_rec* recs, uint32_t cap, uint32_
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------- profiling ----
+
+extern "C" int ep_profile_iteration(ep_ctx* h, const uint32_t* ids, uint32_t n, const ep_config* cfg,
+                                    uint32_t warm, uint32_t reps, ep_phase_time* out, uint32_t cap,
+                                    uint32_t* n_out) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !ids || !cfg || !n_out || (!out && cap)) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    if (c->server_running) return fail(c, EP_ERR_INVALID_ARG, "context is serving; stop it first");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<Design*> ds;
+    for (uint32_t i = 0; i < n; i++) {
+        Design* D = find_design(c, ids[i]);
+        if (!D) return fail(c, EP_ERR_NOT_FOUND, "unknown design id");
+        ds.push_back(D);
+    }
+    int rc;
+    Batch* b = cached_batch(c, ds, cfg, &rc);
+    if (!b) return rc;
+    if ((rc = batch_reset(c, *b, cfg))) return rc;
+    if ((rc = batch_run_iters(c, *b, (int)warm, false))) return rc;
+    std::vector<hipEvent_t> ev(kNumPhases + 1);
+    for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+    std::vector<double> acc(kNumPhases, 0.0);
+    for (uint32_t r = 0; r < reps; r++) {
+        HIPCHK(c, hipEventRecord(ev[0], c->stream));
+        for (int ph = 0; ph < kNumPhases; ph++) {
+            enqueue_phase(c, *b, ph);
+            HIPCHK(c, hipEventRecord(ev[ph + 1], c->stream));
+        }
+        HIPCHK(c, hipEventSynchronize(ev[kNumPhases]));
+        for (int ph = 0; ph < kNumPhases; ph++) {
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, ev[ph], ev[ph + 1]));
+            acc[ph] += ms;
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    *n_out = kNumPhases;
+    for (uint32_t ph = 0; ph < (uint32_t)kNumPhases && ph < cap; ph++) {
+        memset(&out[ph], 0, sizeof(ep_phase_time));
+        strncpy(out[ph].name, kPhaseNames[ph], sizeof(out[ph].name) - 1);
+        out[ph].avg_ms = reps ? acc[ph] / reps : 0.0;
+        out[ph].launches = reps;
+    }
+    return EP_OK;
+}
 
 // -------------------------------------------------------- evaluation -----
 
@@ -533,22 +605,16 @@ int ep_eval_wa_grad(ep_ctx* h, uint32_t id, const double* x, const double* y, do
     const size_t N = (size_t)D->dev.n_mov;
     if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
     if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
-    HIPCHK(c, hipMemsetAsync(D->dev.gwl[0], 0, N * 8, c->stream));
-    HIPCHK(c, hipMemsetAsync(D->dev.gwl[1], 0, N * 8, c->stream));
+    if (!N) return EP_OK;
+    double* dbuf = nullptr;
+    HIPCHK(c, hipMalloc(&dbuf, N * 16));
     launch_wa(b->bd, b->prec, b->wa_blocks, b->hf_blocks, c->stream);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int ax = 0; ax < 2; ax++) {
-        double* out = ax ? gy : gx;
-        if (!N) continue;
-        if (D->prec == EP_PREC_FP64) {
-            HIPCHK(c, hipMemcpy(out, D->dev.gwl[ax], N * 8, hipMemcpyDeviceToHost));
-        } else {
-            std::vector<long long> a(N);
-            HIPCHK(c, hipMemcpy(a.data(), D->dev.acc[ax], N * 8, hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < N; i++) out[i] = (double)a[i] * (1.0 / kFixScale);
-        }
-    }
+    launch_wa_reduce(b->bd, b->prec, b->cell_blocks, dbuf, dbuf + N, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(gx, dbuf, N * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(gy, dbuf + N, N * 8, hipMemcpyDeviceToHost);
+    hipFree(dbuf);
+    if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, hipGetErrorString(e));
     return EP_OK;
 }
 
@@ -564,7 +630,7 @@ int ep_eval_density(ep_ctx* h, uint32_t id, const double* x, const double* y, do
     const size_t N = (size_t)D->dev.n_mov, NB = (size_t)D->dev.Bx * D->dev.By;
     if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
     if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
-    launch_scatter(b->bd, b->prec, b->cell_blocks, c->stream);
+    launch_scatter(b->bd, b->prec, b->scat_blocks, c->stream);
     launch_dens_final(b->bd, b->prec, b->bin_blocks, c->stream);
     HIPCHK(c, hipGetLastError());
     if ((rc = get_T(c, dens, D->dev.dens, NB, D->prec))) return rc;

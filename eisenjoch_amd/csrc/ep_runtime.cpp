@@ -133,20 +133,26 @@ int get_transform(Ctx* c, int n, int mode, Transform** out) {
 
 // Net layout: nets of degree 2..64 go to bucket b (group size G = 2^(b+1),
 // the smallest power of two >= degree); each net occupies G consecutive
-// slots; every bucket segment is padded to whole waves.  Larger nets go to a
-// CSR list handled one block per net.  Nets with fewer than two pins are
-// skipped, as common.rs:242 does.
+// slots; every bucket segment is padded to whole waves.  Inside a bucket,
+// nets are ordered by their smallest movable cell index, so nets of one
+// cluster are neighbours in memory and so are the gradient slots a block of
+// cells reads back.  Larger nets go to a CSR list handled one block per
+// net; their pins take the slots after the wave slots.  Nets with fewer
+// than two pins are skipped, as common.rs:242 does.  The cell -> slot CSR
+// lists, for every movable cell, the slots of its pins in ascending order.
 struct NetLayout {
     std::vector<uint32_t> slots;
     int n_buckets = kMaxBuckets;
     int bucket_wave_start[kMaxBuckets + 1] = {0};
     int bucket_group[kMaxBuckets] = {0};
     std::vector<uint32_t> hf_off, hf_pins;
+    std::vector<uint32_t> cs_off, cs_idx;
 };
 
 static void build_layout(const ep_design* d, NetLayout& L) {
-    std::vector<std::vector<uint32_t>> seg(kMaxBuckets);
+    std::vector<std::vector<std::pair<uint32_t, int>>> nets(kMaxBuckets);
     L.hf_off.assign(1, 0);
+    const uint32_t nm = (uint32_t)d->n_mov;
     for (int e = 0; e < d->n_nets; e++) {
         uint32_t b0 = d->net_off[e], n = d->net_off[e + 1] - b0;
         if (n < 2) continue;
@@ -157,18 +163,45 @@ static void build_layout(const ep_design* d, NetLayout& L) {
         }
         int b = 0;
         while ((2u << b) < n) b++;
-        const uint32_t G = 2u << b;
-        for (uint32_t k = 0; k < G; k++) seg[b].push_back(k < n ? d->pin_cell[b0 + k] : kPad);
+        uint32_t key = 0xFFFFFFFFu;
+        for (uint32_t k = 0; k < n; k++)
+            if (d->pin_cell[b0 + k] < nm) key = std::min(key, d->pin_cell[b0 + k]);
+        nets[b].push_back({key, e});
     }
     int waves = 0;
     for (int b = 0; b < kMaxBuckets; b++) {
+        std::stable_sort(nets[b].begin(), nets[b].end(),
+                         [](const std::pair<uint32_t, int>& x, const std::pair<uint32_t, int>& y) {
+                             return x.first < y.first;
+                         });
         L.bucket_wave_start[b] = waves;
         L.bucket_group[b] = 2 << b;
-        while (seg[b].size() % kWave) seg[b].push_back(kPad);
-        waves += (int)(seg[b].size() / kWave);
-        L.slots.insert(L.slots.end(), seg[b].begin(), seg[b].end());
+        const uint32_t G = 2u << b;
+        const size_t start = L.slots.size();
+        for (auto& kv : nets[b]) {
+            uint32_t b0 = d->net_off[kv.second], n = d->net_off[kv.second + 1] - b0;
+            for (uint32_t k = 0; k < G; k++) L.slots.push_back(k < n ? d->pin_cell[b0 + k] : kPad);
+        }
+        while ((L.slots.size() - start) % kWave) L.slots.push_back(kPad);
+        waves += (int)((L.slots.size() - start) / kWave);
     }
     L.bucket_wave_start[kMaxBuckets] = waves;
+    // cell -> slot CSR (counting sort; slots visited in ascending order)
+    const size_t nws = L.slots.size();
+    L.cs_off.assign((size_t)nm + 1, 0);
+    auto slot_cell = [&](size_t s) -> uint32_t { return s < nws ? L.slots[s] : L.hf_pins[s - nws]; };
+    const size_t ns = nws + L.hf_pins.size();
+    for (size_t s = 0; s < ns; s++) {
+        uint32_t c = slot_cell(s);
+        if (c != kPad && c < nm) L.cs_off[c + 1]++;
+    }
+    for (size_t i = 0; i < nm; i++) L.cs_off[i + 1] += L.cs_off[i];
+    L.cs_idx.resize(L.cs_off[nm]);
+    std::vector<uint32_t> fillp(L.cs_off.begin(), L.cs_off.end() - 1);
+    for (size_t s = 0; s < ns; s++) {
+        uint32_t c = slot_cell(s);
+        if (c != kPad && c < nm) L.cs_idx[fillp[c]++] = (uint32_t)s;
+    }
 }
 
 static int validate_design(Ctx* c, const ep_design* d) {
@@ -227,11 +260,9 @@ static size_t plan_arena(Design& D, char* base) {
     for (int ax = 0; ax < 2; ax++) G.look[ax] = P(N * T);
     for (int p = 0; p < 2; p++)
         for (int ax = 0; ax < 2; ax++) G.g[p][ax] = P(N * T);
-    for (int ax = 0; ax < 2; ax++) {
-        void* q = P(N * 8);
-        G.gwl[ax] = q;
-        G.acc[ax] = (long long*)q;
-    }
+    G.pg = P((size_t)G.n_slots * 2 * T);
+    G.cs_off = (uint32_t*)P((N + 1) * 4);
+    G.cs_idx = (uint32_t*)P(D.cs_count * 4);
     G.pinw = P(N * T);
     G.hs = (float*)P(N * 4);
     for (int ax = 0; ax < 2; ax++) G.fixed[ax] = P(F * T);
@@ -283,6 +314,7 @@ static void fill_desc_sizes(Design& D, const ep_design* d, const NetLayout& L) {
     G.By = d->By > 0 ? d->By : d->H;
     G.prec = D.prec;
     G.n_cell_blocks = (G.n_mov + kBlock - 1) / kBlock;
+    G.n_scat_blocks = (G.n_mov + kScatCells - 1) / kScatCells;
     G.n_bin_blocks = (G.Bx * G.By + kBlock - 1) / kBlock;
     G.n_wa_waves = L.bucket_wave_start[kMaxBuckets];
     G.n_hf = (int)L.hf_off.size() - 1;
@@ -294,6 +326,8 @@ static void fill_desc_sizes(Design& D, const ep_design* d, const NetLayout& L) {
     G.max_x = (double)(G.W - 1);  // mod.rs:73-74
     G.max_y = (double)(G.H - 1);
     D.hf_pin_count = L.hf_pins.size();
+    G.n_slots = G.n_wa_waves * kWave + (int)L.hf_pins.size();
+    D.cs_count = L.cs_idx.size();
 }
 
 }  // namespace ep
@@ -442,6 +476,9 @@ static int fill_arena(Ctx* c, Design* D, const ep_design* d, const NetLayout& L)
     HIPCHK(c, hipMemcpy(D->dev.hf_off, L.hf_off.data(), L.hf_off.size() * 4, hipMemcpyHostToDevice));
     if (!L.hf_pins.empty())
         HIPCHK(c, hipMemcpy(D->dev.hf_pins, L.hf_pins.data(), L.hf_pins.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(D->dev.cs_off, L.cs_off.data(), L.cs_off.size() * 4, hipMemcpyHostToDevice));
+    if (!L.cs_idx.empty())
+        HIPCHK(c, hipMemcpy(D->dev.cs_idx, L.cs_idx.data(), L.cs_idx.size() * 4, hipMemcpyHostToDevice));
     const double pi = 3.14159265358979323846;
     std::vector<double> sx(2 * (size_t)D->dev.Bx), sy(2 * (size_t)D->dev.By);
     for (int k = 0; k < D->dev.Bx; k++) {  // density.rs:139-140

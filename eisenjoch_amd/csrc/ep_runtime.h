@@ -39,6 +39,7 @@ struct Design {
     void* arena = nullptr;
     size_t bytes = 0;
     size_t hf_pin_count = 0;
+    size_t cs_count = 0;            // movable pins (cell -> slot CSR entries)
     DesignDev dev{};
 };
 
@@ -50,7 +51,7 @@ struct Batch {
     std::vector<ep_config> cfgs;
     int prec = EP_PREC_FAST;
     int log_hpwl = 1;
-    int cell_blocks = 0, bin_blocks = 0, wa_blocks = 0, hf_blocks = 0;
+    int cell_blocks = 0, bin_blocks = 0, wa_blocks = 0, hf_blocks = 0, scat_blocks = 0;
     int njobs2[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     int tiles2[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     GemmJob* jobs2[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
@@ -60,6 +61,7 @@ struct Batch {
     BatchDev bd{};
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
+    std::vector<double> done_ms;     // per design: host time at which it was seen done
 };
 
 struct Server;

@@ -65,7 +65,7 @@ static BatchDev single_view(const Batch& b, int i) {
     BatchDev one = b.bd;
     one.d = b.d_dev + i;
     one.nd = 1;
-    one.pfx_cell = one.pfx_bin = one.pfx_wa = one.pfx_hf = b.dzero;
+    one.pfx_cell = one.pfx_bin = one.pfx_wa = one.pfx_hf = one.pfx_scat = b.dzero;
     return one;
 }
 

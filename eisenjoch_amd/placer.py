@@ -390,6 +390,16 @@ class Device:
         keys = [f[0] for f in L.EpIterRec._fields_]
         return [{k.rstrip("_"): getattr(recs[i], k) for k in keys} for i in range(min(n.value, cap))]
 
+    def profile(self, ids: Sequence[int], cfg: ElectroPlaceCfg, warm: int = 8, reps: int = 5) -> dict:
+        """Average HIP-event time (ms) of each phase of one batched iteration."""
+        n = len(ids)
+        arr = (C.c_uint32 * n)(*ids)
+        out = (L.EpPhaseTime * 32)()
+        k = C.c_uint32()
+        _check(L.lib().ep_profile_iteration(self._h, arr, n, C.byref(cfg.to_c()), warm, reps, out, 32,
+                                            C.byref(k)), self._h, "ep_profile_iteration")
+        return {out[i].name.decode(): out[i].avg_ms for i in range(k.value)}
+
     # -- single-kernel evaluation --
     def eval_config(self, cfg: ElectroPlaceCfg):
         _check(L.lib().ep_eval_config(self._h, C.byref(cfg.to_c())), self._h, "ep_eval_config")

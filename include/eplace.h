@@ -183,6 +183,20 @@ int ep_serve_release(ep_ctx* ctx, uint64_t ticket);
 int ep_serve_cancel(ep_ctx* ctx, uint64_t ticket);
 int ep_serve_stop(ep_ctx* ctx);
 
+/* ---- measurement ---- */
+typedef struct ep_phase_time {
+    char name[24];
+    int32_t launches;       /* iterations averaged */
+    int32_t reserved;
+    double avg_ms;          /* HIP-event time of the phase per iteration */
+} ep_phase_time;
+
+/* Resets the listed designs, runs `warm` iterations, then times each phase
+ * of `reps` further iterations with HIP events on the context's stream. */
+int ep_profile_iteration(ep_ctx* ctx, const uint32_t* ids, uint32_t n, const ep_config* cfg,
+                         uint32_t warm, uint32_t reps, ep_phase_time* out, uint32_t cap,
+                         uint32_t* n_out);
+
 /* ---- host helpers ---- */
 /* Greedy legalization, mod.rs:206-256: cells in index order take the free
  * BEL of their bucket nearest in squared distance, first minimum in BEL

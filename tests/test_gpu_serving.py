@@ -1,0 +1,98 @@
+"""Serving: continuous batching on one GPU.  Requests that arrive while
+others run are admitted as slots and memory free up; each result is the same
+alone and in the mix (bitwise on the FAST path, whose sums are fixed-point
+or fixed-order per design)."""
+import time
+
+import numpy as np
+import pytest
+
+import eisenjoch_amd as ep
+from conftest import log_case
+
+pytestmark = pytest.mark.gpu
+
+CFG = ep.ElectroPlaceCfg(max_iters=300, field_mode="idxst", log_hpwl=False)
+
+
+def wait(srv, tickets, timeout=120.0):
+    out = {}
+    t0 = time.time()
+    while len(out) < len(tickets):
+        for t in tickets:
+            if t in out:
+                continue
+            r = srv.poll(t)
+            if r is not None:
+                out[t] = r
+        if time.time() - t0 > timeout:
+            raise TimeoutError(f"{len(out)}/{len(tickets)} requests finished")
+        time.sleep(0.002)
+    return out
+
+
+def test_serving_results_equal_alone():
+    sizes = [1500, 3000, 6000, 2500, 12000, 4000]
+    ds = [ep.sized_design(n, 300 + k) for k, n in enumerate(sizes)]
+    with ep.Device(0) as dev:
+        log_case("serving: alone runs")
+        alone = []
+        for d in ds:
+            i = dev.upload(d, "fast")
+            alone.append(dev.place(i, CFG))
+            dev.free(i)
+        log_case("serving: mixed arrivals, 3 slots")
+        srv = ep.Server(dev, max_slots=3, iters_per_sync=5)
+        try:
+            tickets = []
+            for k, d in enumerate(ds):
+                tickets.append(srv.submit(d, CFG))
+                time.sleep(0.003 * (k % 3))  # staggered arrivals while others run
+            res = wait(srv, tickets)
+            for k, t in enumerate(tickets):
+                r = res[t]
+                assert r.status == 0
+                x, y = srv.positions(t)
+                assert r.iters == alone[k].iters and r.converged == alone[k].converged
+                np.testing.assert_array_equal(x, alone[k].x)
+                np.testing.assert_array_equal(y, alone[k].y)
+                assert r.hpwl == alone[k].hpwl
+                assert r.t_submit_ms <= r.t_admit_ms <= r.t_done_ms
+                srv.release(t)
+        finally:
+            srv.stop()
+
+
+def test_serving_slots_and_cancel():
+    ds = [ep.sized_design(n, 400 + k) for k, n in enumerate([3000, 3000, 3000, 3000])]
+    with ep.Device(0) as dev:
+        log_case("serving: slot limit 1 and cancel")
+        srv = ep.Server(dev, max_slots=1, iters_per_sync=5)
+        try:
+            t = [srv.submit(d, CFG) for d in ds]
+            srv.cancel(t[3])  # still queued behind a single slot
+            res = wait(srv, t)
+            assert res[t[3]].status == ep._lib.EP_ERR_CANCELLED
+            done = [res[k] for k in t[:3]]
+            assert all(r.status == 0 and r.converged for r in done)
+            # one slot: each admission waits for the previous finish
+            assert done[1].t_admit_ms >= done[0].t_done_ms - 1.0
+            assert done[2].t_admit_ms >= done[1].t_done_ms - 1.0
+        finally:
+            srv.stop()
+
+
+def test_serving_admission_refused_over_budget():
+    d = ep.sized_design(20000, 5)
+    with ep.Device(0, hbm_budget_bytes=16 << 20) as dev:
+        log_case("serving: budget refusal")
+        srv = ep.Server(dev, max_slots=4)
+        try:
+            big = ep.sized_design(200_000, 6)
+            with pytest.raises(ep.AdmissionRefused):
+                srv.submit(big, CFG)
+            t = srv.submit(d, CFG)
+            r = wait(srv, [t])[t]
+            assert r.status == 0
+        finally:
+            srv.stop()
