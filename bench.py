@@ -46,6 +46,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--profile-only", action="store_true", help="one step, no baselines (for rocprof)")
+    p.add_argument("--calib-copy", type=int, default=0,
+                   help="profile mode: also copy this many MiB device-to-device with one plain "
+                        "copy kernel (counter calibration)")
     p.add_argument("--serve", type=float, default=0.0,
                    help="serving scenario: Poisson arrival rate in requests/s (0: batch workload)")
     p.add_argument("--duration", type=float, default=120.0, help="serving: arrival window in seconds")
@@ -209,6 +212,15 @@ def main():
                        "p99": float(np.percentile(lat, 99))},
     }
     if args.profile_only:
+        if args.calib_copy:
+            import torch
+            a = torch.empty(args.calib_copy << 18, dtype=torch.float32, device="cuda")
+            b = torch.empty_like(a)
+            a.fill_(1.0)
+            torch.cuda.synchronize()
+            b.copy_(a)
+            torch.cuda.synchronize()
+            out["calib_copy_bytes"] = int(a.numel() * 4)
         print(json.dumps(out), flush=True)
         return
 

@@ -125,31 +125,43 @@ __device__ inline void split8(const float* v, bf16x8& hi, bf16x8& lo) {
     }
 }
 
-// Stage a 64 x 32 operand tile (rows r0.., k0..) into LDS hi/lo planes.
-__device__ inline void stage_tile(__bf16 (*s)[64][kLdk], int kind, const void* p, const void* plo,
-                                  int r0, int k0, int K) {
+// One thread's share of a 64 x 32 operand tile: 8 consecutive K values of
+// one row, held as BF16 hi/lo in registers between the global load and the
+// LDS store (so the next stage's loads overlap this stage's MFMAs).
+struct Frag {
+    bf16x8 hi, lo;
+};
+
+__device__ inline Frag load_frag(int kind, const void* p, const void* plo, int r0, int k0, int K) {
     const int tid = threadIdx.x;
     const int row = tid >> 2, kc = (tid & 3) * 8;
     const size_t off = (size_t)(r0 + row) * K + k0 + kc;
-    bf16x8 hi, lo;
+    Frag f;
     if (kind == 0) {
         const float4* src = (const float4*)((const float*)p + off);
-        float4 a = src[0], b = src[1];
-        float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        split8(v, hi, lo);
+        const float4 a = src[0], b = src[1];
+        const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        split8(v, f.hi, f.lo);
     } else {
-        hi = *(const bf16x8*)((const __bf16*)p + off);
-        lo = *(const bf16x8*)((const __bf16*)plo + off);
+        f.hi = *(const bf16x8*)((const __bf16*)p + off);
+        f.lo = *(const bf16x8*)((const __bf16*)plo + off);
     }
-    *(bf16x8*)&s[0][row][kc] = hi;
-    *(bf16x8*)&s[1][row][kc] = lo;
+    return f;
 }
 
-// 64 x 64 output tile per block, four waves each owning a 32 x 32 quadrant.
-// Requires M, N multiples of 64 and K a multiple of 32 (checked on the host).
+__device__ inline void store_frag(__bf16 (*s)[64][kLdk], const Frag& f) {
+    const int tid = threadIdx.x;
+    const int row = tid >> 2, kc = (tid & 3) * 8;
+    *(bf16x8*)&s[0][row][kc] = f.hi;
+    *(bf16x8*)&s[1][row][kc] = f.lo;
+}
+
+// 64 x 64 output tile per block, four waves each owning a 32 x 32 quadrant,
+// K in stages of 32 through two LDS buffers.  Requires M, N multiples of 64
+// and K a multiple of 32 (checked on the host).
 __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, const DesignDev* dd) {
-    __shared__ __bf16 sA[2][64][kLdk];
-    __shared__ __bf16 sB[2][64][kLdk];
+    __shared__ __bf16 sA[2][2][64][kLdk];  // [buffer][hi/lo][row][k]
+    __shared__ __bf16 sB[2][2][64][kLdk];
     const int j = find_job(jobs, nj, blockIdx.x);
     const GemmJob& J = jobs[j];
     const DesignDev& D = dd[J.design];
@@ -158,23 +170,40 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, 
     const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int wm = wv >> 1, wn = wv & 1;
+    const void* pa = J.a_kind ? J.A_hi : J.A;
+    const void* pb = J.b_kind ? J.B_hi : J.B;
     f32x16 acc;
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = 0.0f;
     const int r = lane & 31, kh = (lane >> 5) * 8;
-    for (int k0 = 0; k0 < J.K; k0 += kTK) {
-        stage_tile(sA, J.a_kind, J.a_kind ? J.A_hi : J.A, J.A_lo, m0, k0, J.K);
-        stage_tile(sB, J.b_kind, J.b_kind ? J.B_hi : J.B, J.B_lo, n0, k0, J.K);
-        __syncthreads();
+    Frag fa = load_frag(J.a_kind, pa, J.A_lo, m0, 0, J.K);
+    Frag fb = load_frag(J.b_kind, pb, J.B_lo, n0, 0, J.K);
+    store_frag(sA[0], fa);
+    store_frag(sB[0], fb);
+    __syncthreads();
+    const int stages = J.K / kTK;
+    for (int st = 0; st < stages; st++) {
+        const int cur = st & 1;
+        const bool more = st + 1 < stages;
+        if (more) {
+            fa = load_frag(J.a_kind, pa, J.A_lo, m0, (st + 1) * kTK, J.K);
+            fb = load_frag(J.b_kind, pb, J.B_lo, n0, (st + 1) * kTK, J.K);
+        }
 #pragma unroll
         for (int kk = 0; kk < kTK; kk += 16) {
-            const bf16x8 ah = *(const bf16x8*)&sA[0][wm * 32 + r][kk + kh];
-            const bf16x8 al = *(const bf16x8*)&sA[1][wm * 32 + r][kk + kh];
-            const bf16x8 bh = *(const bf16x8*)&sB[0][wn * 32 + r][kk + kh];
-            const bf16x8 bl = *(const bf16x8*)&sB[1][wn * 32 + r][kk + kh];
+            const bf16x8 ah = *(const bf16x8*)&sA[cur][0][wm * 32 + r][kk + kh];
+            const bf16x8 al = *(const bf16x8*)&sA[cur][1][wm * 32 + r][kk + kh];
+            const bf16x8 bh = *(const bf16x8*)&sB[cur][0][wn * 32 + r][kk + kh];
+            const bf16x8 bl = *(const bf16x8*)&sB[cur][1][wn * 32 + r][kk + kh];
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+        }
+        if (more) {
+            // the other buffer was last read in stage st - 1, before the
+            // barrier that ended it, so it is free to overwrite
+            store_frag(sA[cur ^ 1], fa);
+            store_frag(sB[cur ^ 1], fb);
         }
         __syncthreads();
     }

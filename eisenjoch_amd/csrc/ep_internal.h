@@ -62,7 +62,12 @@ struct DevCfg {
 struct DesignDev {
     int32_t n_mov, n_fixed, n_nets, W, H, Bx, By, prec;
     int32_t n_cell_blocks, n_bin_blocks, n_wa_waves, n_hf, n_buckets, n_scat_blocks;
-    int32_t n_slots, pad1;        // per-pin gradient slots: n_wa_waves * 64 + high-degree pins
+    int32_t n_slots;              // per-pin gradient slots: n_wa_waves * 64 + high-degree pins
+    int32_t n_keys;               // Morton keys of the 8x8-bin tiles (a power of four)
+    int32_t tiles_x, tiles_y;     // 8x8-bin tiles covering the bin grid
+    int32_t scat_pull;            // 1: tile-owned pull scatter (footprints span <= 9 bins)
+    int32_t n_tile_blocks;        // pull scatter blocks (tiles_x * tiles_y, or 0)
+    int32_t n_ov_parts;           // overflow partial pairs written per iteration
     int32_t bucket_wave_start[kMaxBuckets + 1];
     int32_t bucket_group[kMaxBuckets];
     double rx, ry, max_x, max_y;
@@ -78,6 +83,11 @@ struct DesignDev {
     void* pg;                  // per-slot WA gradient (gx, gy) in T [n_slots][2]
     uint32_t* cs_off;          // cell -> slot CSR [n_mov + 1]
     uint32_t* cs_idx;          // slots of each movable cell, ascending
+    uint32_t* perm[2];         // cells in Morton order of their lower-left bin tile; the
+                               // sort of iteration k reads perm[k&1] and writes perm[(k+1)&1]
+    uint32_t* tcount;          // [n_keys] cells per tile key (zero between sorts)
+    uint32_t* tstart;          // [n_keys + 1] first position of each key in perm
+    uint32_t* tcur;            // [n_keys] fill cursors
     void* pinw;                // T[n_mov]
     float* hs;                 // half side of the footprint [n_mov]
     void* fixed[2];            // T[n_fixed]
@@ -131,7 +141,8 @@ struct BatchDev {
     const int32_t* pfx_bin;
     const int32_t* pfx_wa;     // blocks of 4 waves
     const int32_t* pfx_hf;     // one block per high-degree net
-    const int32_t* pfx_scat;   // scatter blocks (kScatCells cells each)
+    const int32_t* pfx_scat;   // atomic scatter blocks (kScatCells cells each)
+    const int32_t* pfx_tile;   // pull scatter blocks (one per 8x8-bin tile)
 };
 
 // ---- launchers (ep_kernels.hip) ----
@@ -145,6 +156,8 @@ void launch_update(const BatchDev& b, int prec, int total_cell_blocks, hipStream
 void launch_hpwl(const BatchDev& b, int prec, int total_wa_blocks, int total_hf_blocks,
                  int final_pass, hipStream_t s);
 void launch_reset(const BatchDev& b, int prec, int total_cell_blocks, hipStream_t s);
+void launch_tile_sort(const BatchDev& b, int prec, int total_cell_blocks, hipStream_t s);
+void launch_scatter_pull(const BatchDev& b, int prec, int total_tile_blocks, hipStream_t s);
 void launch_bin_ranges(const BatchDev& b, int total_cell_blocks, int32_t* out, hipStream_t s);
 void launch_gather_export(const BatchDev& b, int prec, int total_cell_blocks, double* gx,
                           double* gy, hipStream_t s);

@@ -286,12 +286,13 @@ __device__ inline double ov_area(const BinBox& r, int bx, int by) {
     return ox * oy;
 }
 
-// One block covers kScatCells consecutive cells.  Cells of a cluster have
-// neighbouring indices and sit close together once placement has started,
-// so when the block's bin bounding box fits kScatBins the contributions are
-// summed in LDS and flushed with one global atomic per touched bin;
-// otherwise each contribution goes to global memory directly.  All sums are
-// fixed point (FAST) so the result does not depend on arrival order.
+// One block covers kScatCells consecutive cells of the tile order `perm`
+// (refreshed every sync interval by the tile sort below), so its cells sit
+// in a compact group of bins; when the block's bin bounding box fits
+// kScatBins the contributions are summed in LDS and flushed with one global
+// atomic per touched bin, otherwise each contribution goes to global memory
+// directly.  All sums are fixed point (FAST), so neither the order nor the
+// path changes the result.
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
     typedef typename std::conditional<FIX, unsigned long long, double>::type Acc;
@@ -301,6 +302,7 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
     const int c0 = (blockIdx.x - b.pfx_scat[d]) * kScatCells;
+    const uint32_t* PERM = D.perm[(D.st->it + 1) & 1];
     const T* LX = (const T*)D.look[0];
     const T* LY = (const T*)D.look[1];
     if (threadIdx.x == 0) {
@@ -310,8 +312,8 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
     __syncthreads();
     int lo_x = 0x7fffffff, hi_x = -1, lo_y = 0x7fffffff, hi_y = -1;
     for (int k = threadIdx.x; k < kScatCells; k += blockDim.x) {
-        const int i = c0 + k;
-        if (i >= D.n_mov) break;
+        if (c0 + k >= D.n_mov) break;
+        const int i = PERM[c0 + k];
         const BinBox r = bin_box(D, (double)LX[i], (double)LY[i], (double)D.hs[i]);
         if (r.bx1 > r.bx0 && r.by1 > r.by0) {
             lo_x = min(lo_x, r.bx0); hi_x = max(hi_x, r.bx1);
@@ -328,8 +330,8 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
         for (int k = threadIdx.x; k < ww * hh; k += blockDim.x) win[k] = Acc(0);
     __syncthreads();
     for (int k = threadIdx.x; k < kScatCells; k += blockDim.x) {
-        const int i = c0 + k;
-        if (i >= D.n_mov) break;
+        if (c0 + k >= D.n_mov) break;
+        const int i = PERM[c0 + k];
         const BinBox r = bin_box(D, (double)LX[i], (double)LY[i], (double)D.hs[i]);
         for (int by = r.by0; by < r.by1; by++)
             for (int bx = r.bx0; bx < r.bx1; bx++) {
@@ -377,6 +379,171 @@ __global__ __launch_bounds__(256) void k_dens_final(BatchDev b) {
     if (threadIdx.x == 0) {
         D.part_bin[(size_t)blk * kBinParts + 0] = p[0];
         D.part_bin[(size_t)blk * kBinParts + 1] = p[1];
+    }
+}
+
+// ------------------------------------------------------- tile sort -----
+
+// Morton interleave of two 12-bit tile coordinates.
+__device__ inline uint32_t morton2(uint32_t x, uint32_t y) {
+    x = (x | (x << 8)) & 0x00FF00FFu; x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u; x = (x | (x << 1)) & 0x55555555u;
+    y = (y | (y << 8)) & 0x00FF00FFu; y = (y | (y << 4)) & 0x0F0F0F0Fu;
+    y = (y | (y << 2)) & 0x33333333u; y = (y | (y << 1)) & 0x55555555u;
+    return x | (y << 1);
+}
+
+template <typename T>
+__device__ inline uint32_t tile_key(const DesignDev& D, int i) {
+    const BinBox r = bin_box(D, (double)((const T*)D.look[0])[i], (double)((const T*)D.look[1])[i],
+                             (double)D.hs[i]);
+    const int bx = min(r.bx0, D.Bx - 1), by = min(r.by0, D.By - 1);
+    const uint32_t k = morton2((uint32_t)bx >> 3, (uint32_t)by >> 3);
+    return k < (uint32_t)D.n_keys ? k : (uint32_t)D.n_keys - 1;
+}
+
+// One atomic per distinct key of a wave: lanes with equal keys are counted
+// together (the previous order keeps them mostly equal); with want_pos each
+// lane also receives its own slot (base + rank among the lanes of its key).
+__device__ inline uint32_t wave_agg_add(uint32_t* ctr, uint32_t key, bool valid) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long act = __ballot(valid);
+    uint32_t mine = 0;
+    while (act) {
+        const int l = __ffsll((long long)act) - 1;
+        const uint32_t lk = __shfl(key, l, kWave);
+        const unsigned long long same = __ballot(valid && key == lk) & act;
+        uint32_t base = 0;
+        if (lane == l) base = atomicAdd(&ctr[lk], (uint32_t)__popcll(same));
+        base = __shfl(base, l, kWave);
+        if ((same >> lane) & 1ull) mine = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        act &= ~same;
+    }
+    return mine;
+}
+
+// Counting sort of the cells by tile key, every iteration: count, scan, fill.
+// Cells are visited in the previous order, so a wave sees few distinct keys.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
+    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    if (D.st->done) return;
+    const int k = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    const bool valid = k < D.n_mov;
+    const uint32_t* src = D.perm[D.st->it & 1];
+    const uint32_t key = valid ? tile_key<T>(D, (int)src[k]) : 0u;
+    wave_agg_add(D.tcount, key, valid);
+}
+
+// One block per design: tstart = exclusive scan of the counts (with the
+// total at n_keys), cursors = tstart, counts cleared for the next sort.
+__global__ __launch_bounds__(256) void k_tile_scan(BatchDev b) {
+    __shared__ uint32_t part[256];
+    const DesignDev& D = b.d[blockIdx.x];
+    if (D.st->done) return;
+    const int n = D.n_keys;
+    const int per = (n + 255) / 256;
+    const int lo = min(threadIdx.x * per, n), hi = min(lo + per, n);
+    uint32_t sum = 0;
+    for (int k = lo; k < hi; k++) sum += D.tcount[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int t = 0; t < 256; t++) {
+            const uint32_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+        D.tstart[n] = run;
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x];
+    for (int k = lo; k < hi; k++) {
+        const uint32_t v = D.tcount[k];
+        D.tstart[k] = run;
+        D.tcur[k] = run;
+        D.tcount[k] = 0u;
+        run += v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tile_fill(BatchDev b) {
+    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    if (D.st->done) return;
+    const int k = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    const bool valid = k < D.n_mov;
+    const int par = D.st->it & 1;
+    const uint32_t i = valid ? D.perm[par][k] : 0u;
+    const uint32_t key = valid ? tile_key<T>(D, (int)i) : 0u;
+    const uint32_t pos = wave_agg_add(D.tcur, key, valid);
+    if (valid && pos < (uint32_t)D.n_mov) D.perm[par ^ 1][pos] = i;
+}
+
+// Pull scatter: one block per 8x8-bin tile.  Every cell whose footprint
+// reaches the tile has its lower-left bin in this tile or the tile to the
+// left, below, or below-left (footprints span at most 9 bins), so the block
+// reads those four key ranges of the sorted order, sums the overlaps that
+// fall inside its own 64 bins in LDS (fixed point on the FAST path, hence
+// independent of order), and writes the density and its overflow partials
+// with plain stores: no global atomics.
+template <typename T, bool FIX>
+__global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
+    typedef typename std::conditional<FIX, unsigned long long, double>::type Acc;
+    __shared__ Acc win[64];
+    __shared__ double sh[4 * kBinParts];
+    const int d = find_blk(b.pfx_tile, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    if (D.st->done) return;
+    const int t = blockIdx.x - b.pfx_tile[d];
+    const int tx = t % D.tiles_x, ty = t / D.tiles_x;
+    const int x0 = tx * 8, y0 = ty * 8;
+    if (threadIdx.x < 64) win[threadIdx.x] = Acc(0);
+    __syncthreads();
+    const uint32_t* PERM = D.perm[(D.st->it + 1) & 1];
+    const T* LX = (const T*)D.look[0];
+    const T* LY = (const T*)D.look[1];
+    for (int q = 0; q < 4; q++) {
+        const int sx = tx - (q & 1), sy = ty - (q >> 1);
+        if (sx < 0 || sy < 0) continue;
+        const uint32_t key = morton2((uint32_t)sx, (uint32_t)sy);
+        if (key >= (uint32_t)D.n_keys) continue;
+        const uint32_t s0 = D.tstart[key], s1 = D.tstart[key + 1];
+        for (uint32_t k = s0 + threadIdx.x; k < s1; k += blockDim.x) {
+            const int i = PERM[k];
+            const BinBox r = bin_box(D, (double)LX[i], (double)LY[i], (double)D.hs[i]);
+            const int bx0 = max(r.bx0, x0), bx1 = min(r.bx1, x0 + 8);
+            const int by0 = max(r.by0, y0), by1 = min(r.by1, y0 + 8);
+            for (int by = by0; by < by1; by++)
+                for (int bx = bx0; bx < bx1; bx++) {
+                    const double w = ov_area(r, bx, by);
+                    Acc v;
+                    if constexpr (FIX) v = (Acc)__double2ll_rn(w * kFixScale);
+                    else v = w;
+                    atomicAdd(&win[(by - y0) * 8 + (bx - x0)], v);
+                }
+        }
+    }
+    __syncthreads();
+    double p[kBinParts] = {0.0, 0.0};
+    if (threadIdx.x < 64) {
+        const int bx = x0 + (threadIdx.x & 7), by = y0 + (threadIdx.x >> 3);
+        if (bx < D.Bx && by < D.By) {
+            double v;
+            if constexpr (FIX) v = (double)(long long)win[threadIdx.x] * (1.0 / kFixScale);
+            else v = win[threadIdx.x];
+            ((T*)D.dens)[(size_t)by * D.Bx + bx] = (T)v;
+            p[0] = v;
+            p[1] = v > 1.0 ? v - 1.0 : 0.0;
+        }
+    }
+    block_sum<kBinParts>(p, sh);
+    if (threadIdx.x == 0) {
+        D.part_bin[(size_t)t * kBinParts + 0] = p[0];
+        D.part_bin[(size_t)t * kBinParts + 1] = p[1];
     }
 }
 
@@ -537,8 +704,8 @@ __global__ __launch_bounds__(256) void k_ctrl(BatchDev b, int which) {
     if (which == 0) {
         reduce_prev(D, st, sh);
         double tot, ovs, p0, p1, p2, p3;
-        bsum_range(D.part_bin, D.n_bin_blocks, kBinParts, 0, tot, sh);
-        bsum_range(D.part_bin, D.n_bin_blocks, kBinParts, 1, ovs, sh);
+        bsum_range(D.part_bin, D.n_ov_parts, kBinParts, 0, tot, sh);
+        bsum_range(D.part_bin, D.n_ov_parts, kBinParts, 1, ovs, sh);
         bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 0, p0, sh);
         bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 1, p1, sh);
         bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 2, p2, sh);
@@ -670,6 +837,7 @@ __global__ __launch_bounds__(256) void k_reset(BatchDev b) {
     const DesignDev& D = b.d[d];
     const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
+    D.perm[0][i] = (uint32_t)i;
 #pragma unroll
     for (int ax = 0; ax < 2; ax++) {
         const T hi = (T)(ax == 0 ? D.max_x : D.max_y);
@@ -811,6 +979,27 @@ void launch_reset(const BatchDev& b, int prec, int cell_blocks, hipStream_t s) {
         hipLaunchKernelGGL((k_reset<double, false>), dim3(cell_blocks), dim3(256), 0, s, b);
     else
         hipLaunchKernelGGL((k_reset<float, true>), dim3(cell_blocks), dim3(256), 0, s, b);
+}
+
+void launch_tile_sort(const BatchDev& b, int prec, int cell_blocks, hipStream_t s) {
+    if (cell_blocks <= 0) return;
+    if (prec == EP_PREC_FP64) {
+        hipLaunchKernelGGL(k_tile_count<double>, dim3(cell_blocks), dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_tile_scan, dim3(b.nd), dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_tile_fill<double>, dim3(cell_blocks), dim3(256), 0, s, b);
+    } else {
+        hipLaunchKernelGGL(k_tile_count<float>, dim3(cell_blocks), dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_tile_scan, dim3(b.nd), dim3(256), 0, s, b);
+        hipLaunchKernelGGL(k_tile_fill<float>, dim3(cell_blocks), dim3(256), 0, s, b);
+    }
+}
+
+void launch_scatter_pull(const BatchDev& b, int prec, int tile_blocks, hipStream_t s) {
+    if (tile_blocks <= 0) return;
+    if (prec == EP_PREC_FP64)
+        hipLaunchKernelGGL((k_scatter_pull<double, false>), dim3(tile_blocks), dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL((k_scatter_pull<float, true>), dim3(tile_blocks), dim3(256), 0, s, b);
 }
 
 void launch_bin_ranges(const BatchDev& b, int cell_blocks, int32_t* out, hipStream_t s) {

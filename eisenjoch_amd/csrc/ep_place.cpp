@@ -65,8 +65,8 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         if (cfgs[i]->log_hpwl) b.log_hpwl = 1;
     }
     std::vector<DesignDev> dv(nd);
-    std::vector<int32_t> pc(nd + 1), pb(nd + 1), pw(nd + 1), ph(nd + 1), ps(nd + 1);
-    pc[0] = pb[0] = pw[0] = ph[0] = ps[0] = 0;
+    std::vector<int32_t> pc(nd + 1), pb(nd + 1), pw(nd + 1), ph(nd + 1), ps(nd + 1), pt(nd + 1);
+    pc[0] = pb[0] = pw[0] = ph[0] = ps[0] = pt[0] = 0;
     // jobs: [pass][engine]
     std::vector<GemmJob> jobs[4][2];
     for (int i = 0; i < nd; i++) {
@@ -78,6 +78,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         pw[i + 1] = pw[i] + (D.dev.n_wa_waves + 3) / 4;
         ph[i + 1] = ph[i] + D.dev.n_hf;
         ps[i + 1] = ps[i] + D.dev.n_scat_blocks;
+        pt[i + 1] = pt[i] + D.dev.n_tile_blocks;
         Transform *tx, *ty;
         int rc = get_transform(c, D.dev.Bx, cfgs[i]->field_mode, &tx);
         if (!rc) rc = get_transform(c, D.dev.By, cfgs[i]->field_mode, &ty);
@@ -127,8 +128,9 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     b.wa_blocks = pw[nd];
     b.hf_blocks = ph[nd];
     b.scat_blocks = ps[nd];
+    b.tile_blocks = pt[nd];
     // device image: descriptors, prefixes, jobs
-    size_t bytes = align_up(sizeof(DesignDev) * nd) + 5 * align_up(sizeof(int32_t) * (nd + 1)) + align_up(4);
+    size_t bytes = align_up(sizeof(DesignDev) * nd) + 6 * align_up(sizeof(int32_t) * (nd + 1)) + align_up(4);
     for (int p = 0; p < 4; p++)
         for (int e = 0; e < 2; e++) {
             int t0 = 0;
@@ -157,6 +159,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     b.bd.pfx_wa = (const int32_t*)put(pw.data(), 4 * (nd + 1));
     b.bd.pfx_hf = (const int32_t*)put(ph.data(), 4 * (nd + 1));
     b.bd.pfx_scat = (const int32_t*)put(ps.data(), 4 * (nd + 1));
+    b.bd.pfx_tile = (const int32_t*)put(pt.data(), 4 * (nd + 1));
     const int32_t zero = 0;
     b.dzero = (const int32_t*)put(&zero, 4);
     for (int p = 0; p < 4; p++)
@@ -237,23 +240,28 @@ static void enqueue_gemm(Ctx* c, Batch& b, int p) {
 }
 
 // Phases of one iteration, in launch order (names used by the profiler).
-static const char* kPhaseNames[] = {"wa", "scatter", "dens_final", "dct_f1", "dct_f2", "dct_i1",
-                                    "dct_i2", "gather", "ctrl_a", "fix", "ctrl_b", "update", "hpwl"};
-constexpr int kNumPhases = 13;
+static const char* kPhaseNames[] = {"sort", "wa", "scatter", "dens_final", "dct_f1", "dct_f2",
+                                    "dct_i1", "dct_i2", "gather", "ctrl_a", "fix", "ctrl_b",
+                                    "update", "hpwl"};
+constexpr int kNumPhases = 14;
 
 static void enqueue_phase(Ctx* c, Batch& b, int ph) {
     hipStream_t s = c->stream;
     switch (ph) {
-        case 0: launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, s); break;  // mod.rs:110-115
-        case 1: launch_scatter(b.bd, b.prec, b.scat_blocks, s); break;        // mod.rs:118
-        case 2: launch_dens_final(b.bd, b.prec, b.bin_blocks, s); break;      // mod.rs:119
-        case 3: case 4: case 5: case 6: enqueue_gemm(c, b, ph - 3); break;   // mod.rs:121-123
-        case 7: launch_gather(b.bd, b.prec, b.cell_blocks, 0, s); break;      // mod.rs:126-156
-        case 8: launch_ctrl(b.bd, 0, s); break;                               // mod.rs:131-143, 159-166
-        case 9: launch_gather(b.bd, b.prec, b.cell_blocks, 1, s); break;
-        case 10: launch_ctrl(b.bd, 1, s); break;                              // nesterov.rs:41-47, mod.rs:176-195
-        case 11: launch_update(b.bd, b.prec, b.cell_blocks, s); break;        // mod.rs:169-173, 105-107
-        case 12: if (b.log_hpwl) launch_hpwl(b.bd, b.prec, b.wa_blocks, b.hf_blocks, 0, s); break;
+        case 0: launch_tile_sort(b.bd, b.prec, b.cell_blocks, s); break;      // scatter order
+        case 1: launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, s); break;  // mod.rs:110-115
+        case 2:                                                               // mod.rs:118-119
+            launch_scatter_pull(b.bd, b.prec, b.tile_blocks, s);
+            launch_scatter(b.bd, b.prec, b.scat_blocks, s);
+            break;
+        case 3: launch_dens_final(b.bd, b.prec, b.bin_blocks, s); break;      // mod.rs:119
+        case 4: case 5: case 6: case 7: enqueue_gemm(c, b, ph - 4); break;   // mod.rs:121-123
+        case 8: launch_gather(b.bd, b.prec, b.cell_blocks, 0, s); break;      // mod.rs:126-156
+        case 9: launch_ctrl(b.bd, 0, s); break;                               // mod.rs:131-143, 159-166
+        case 10: launch_gather(b.bd, b.prec, b.cell_blocks, 1, s); break;
+        case 11: launch_ctrl(b.bd, 1, s); break;                              // nesterov.rs:41-47, mod.rs:176-195
+        case 12: launch_update(b.bd, b.prec, b.cell_blocks, s); break;        // mod.rs:169-173, 105-107
+        case 13: if (b.log_hpwl) launch_hpwl(b.bd, b.prec, b.wa_blocks, b.hf_blocks, 0, s); break;
     }
 }
 
@@ -630,15 +638,17 @@ int ep_eval_density(ep_ctx* h, uint32_t id, const double* x, const double* y, do
     const size_t N = (size_t)D->dev.n_mov, NB = (size_t)D->dev.Bx * D->dev.By;
     if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
     if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
+    launch_tile_sort(b->bd, b->prec, b->cell_blocks, c->stream);
+    launch_scatter_pull(b->bd, b->prec, b->tile_blocks, c->stream);
     launch_scatter(b->bd, b->prec, b->scat_blocks, c->stream);
     launch_dens_final(b->bd, b->prec, b->bin_blocks, c->stream);
     HIPCHK(c, hipGetLastError());
     if ((rc = get_T(c, dens, D->dev.dens, NB, D->prec))) return rc;
     if (overflow) {
-        std::vector<double> p((size_t)D->dev.n_bin_blocks * kBinParts);
+        std::vector<double> p((size_t)D->dev.n_ov_parts * kBinParts);
         HIPCHK(c, hipMemcpy(p.data(), D->dev.part_bin, p.size() * 8, hipMemcpyDeviceToHost));
         double tot = 0, ov = 0;
-        for (int k = 0; k < D->dev.n_bin_blocks; k++) {
+        for (int k = 0; k < D->dev.n_ov_parts; k++) {
             tot += p[(size_t)k * kBinParts];
             ov += p[(size_t)k * kBinParts + 1];
         }

@@ -263,6 +263,11 @@ static size_t plan_arena(Design& D, char* base) {
     G.pg = P((size_t)G.n_slots * 2 * T);
     G.cs_off = (uint32_t*)P((N + 1) * 4);
     G.cs_idx = (uint32_t*)P(D.cs_count * 4);
+    G.perm[0] = (uint32_t*)P(N * 4);
+    G.perm[1] = (uint32_t*)P(N * 4);
+    G.tcount = (uint32_t*)P((size_t)G.n_keys * 4);
+    G.tstart = (uint32_t*)P(((size_t)G.n_keys + 1) * 4);
+    G.tcur = (uint32_t*)P((size_t)G.n_keys * 4);
     G.pinw = P(N * T);
     G.hs = (float*)P(N * 4);
     for (int ax = 0; ax < 2; ax++) G.fixed[ax] = P(F * T);
@@ -278,7 +283,7 @@ static size_t plan_arena(Design& D, char* base) {
     G.spec_x = P((size_t)2 * G.Bx * T);
     G.spec_y = P((size_t)2 * G.By * T);
     G.part_cell = (double*)P((size_t)G.n_cell_blocks * kCellParts * 8);
-    G.part_bin = (double*)P((size_t)G.n_bin_blocks * kBinParts * 8);
+    G.part_bin = (double*)P((size_t)std::max(G.n_ov_parts, 1) * kBinParts * 8);
     G.part_net = (double*)P((size_t)(G.n_wa_waves + G.n_hf + 1) * 8);
     G.log = (ep_iter_rec*)P((size_t)D.log_cap * sizeof(ep_iter_rec));
     return a.off;
@@ -327,6 +332,30 @@ static void fill_desc_sizes(Design& D, const ep_design* d, const NetLayout& L) {
     G.max_y = (double)(G.H - 1);
     D.hf_pin_count = L.hf_pins.size();
     G.n_slots = G.n_wa_waves * kWave + (int)L.hf_pins.size();
+    {
+        G.tiles_x = (G.Bx + 7) / 8;
+        G.tiles_y = (G.By + 7) / 8;
+        const int tiles = std::max(G.tiles_x, G.tiles_y);
+        int side = 1;
+        while (side < tiles) side <<= 1;
+        G.n_keys = side * side;
+        // the pull scatter needs every footprint to span at most 9 bins per
+        // axis (2 h r + 2 <= 9 with margin); wider cells use the atomic scatter
+        double hs_max = 0.5;
+        if (d->area)
+            for (int i = 0; i < d->n_mov; i++)
+                hs_max = std::max(hs_max, 0.5 * std::sqrt((double)d->area[i]));
+        G.scat_pull = (2.0 * hs_max * std::max(G.rx, G.ry) <= 6.5) ? 1 : 0;
+        if (G.scat_pull) {
+            G.n_tile_blocks = G.tiles_x * G.tiles_y;
+            G.n_ov_parts = G.n_tile_blocks;
+            G.n_scat_blocks = 0;
+            G.n_bin_blocks = 0;
+        } else {
+            G.n_tile_blocks = 0;
+            G.n_ov_parts = G.n_bin_blocks;
+        }
+    }
     D.cs_count = L.cs_idx.size();
 }
 
@@ -477,6 +506,13 @@ static int fill_arena(Ctx* c, Design* D, const ep_design* d, const NetLayout& L)
     if (!L.hf_pins.empty())
         HIPCHK(c, hipMemcpy(D->dev.hf_pins, L.hf_pins.data(), L.hf_pins.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(D->dev.cs_off, L.cs_off.data(), L.cs_off.size() * 4, hipMemcpyHostToDevice));
+    {
+        // scatter order: any permutation of the cells is valid; start with the identity
+        std::vector<uint32_t> id(N);
+        for (size_t i = 0; i < N; i++) id[i] = (uint32_t)i;
+        if (N) HIPCHK(c, hipMemcpy(D->dev.perm[0], id.data(), N * 4, hipMemcpyHostToDevice));
+        if (N) HIPCHK(c, hipMemcpy(D->dev.perm[1], id.data(), N * 4, hipMemcpyHostToDevice));
+    }
     if (!L.cs_idx.empty())
         HIPCHK(c, hipMemcpy(D->dev.cs_idx, L.cs_idx.data(), L.cs_idx.size() * 4, hipMemcpyHostToDevice));
     const double pi = 3.14159265358979323846;

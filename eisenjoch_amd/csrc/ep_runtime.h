@@ -52,6 +52,7 @@ struct Batch {
     int prec = EP_PREC_FAST;
     int log_hpwl = 1;
     int cell_blocks = 0, bin_blocks = 0, wa_blocks = 0, hf_blocks = 0, scat_blocks = 0;
+    int tile_blocks = 0;
     int njobs2[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     int tiles2[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     GemmJob* jobs2[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};

@@ -110,6 +110,25 @@ def test_density_and_overflow_parity(device, designs, prec, name):
         device.free(i)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fast"])
+def test_density_large_cells_atomic_path(device, prec):
+    # macro-sized cells (footprints wider than 9 bins) take the atomic scatter
+    d = ep.generate(4000, 4400, 90, n_fixed=16, Bx=128, By=128, area_mix=True, seed=12)
+    d.area = d.area.copy()
+    d.area[:40] = 30.0
+    log_case(f"density atomic path {prec}")
+    i = device.upload(d, prec)
+    try:
+        x, y = rand_pos(d, 13)
+        dens, ov = device.density(i, x, y)
+        od = O.density(d, x, y)
+        tol = 1e-13 if prec == "fp64" else 2e-7
+        np.testing.assert_allclose(dens, od, rtol=tol, atol=tol)
+        assert ov == pytest.approx(O.overlap(od), rel=1e-8)
+    finally:
+        device.free(i)
+
+
 def closed_form(W, H, p, q, mode):
     xs = np.arange(W); ys = np.arange(H)
     cx = np.cos(np.pi * p * (2 * xs + 1) / (2 * W))
