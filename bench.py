@@ -213,14 +213,12 @@ def main():
     }
     if args.profile_only:
         if args.calib_copy:
-            import torch
-            a = torch.empty(args.calib_copy << 18, dtype=torch.float32, device="cuda")
-            b = torch.empty_like(a)
-            a.fill_(1.0)
-            torch.cuda.synchronize()
-            b.copy_(a)
-            torch.cuda.synchronize()
-            out["calib_copy_bytes"] = int(a.numel() * 4)
+            from eisenjoch_amd import _lib
+            nbytes = args.calib_copy << 20
+            rc = _lib.lib().ep_calibrate_copy(dev.handle, nbytes)
+            if rc != 0:
+                raise RuntimeError(f"ep_calibrate_copy failed: {rc}")
+            out["calib_copy_bytes"] = nbytes
         print(json.dumps(out), flush=True)
         return
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 
 #include "ep_internal.h"
 
@@ -216,6 +217,81 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, 
     }
 }
 
+// Split-K variant: the four waves of a block each compute the whole 64 x 64
+// tile over one quarter of K, with MFMA operands loaded straight from
+// global memory (each lane's 8 consecutive K values of one row are 32 bytes
+// of FP32 data or 16 + 16 bytes of pre-split BF16); the partial tiles are
+// summed in LDS in wave order, so the result is deterministic.  The K chain
+// each wave walks is a quarter as long as the single-tile kernel's.
+// Requires M, N multiples of 64 and K a multiple of 64.
+__device__ inline void frag_direct(int kind, const void* p, const void* plo, int row, int k, int K,
+                                   bf16x8& hi, bf16x8& lo) {
+    const size_t off = (size_t)row * K + k;
+    if (kind == 0) {
+        const float4* src = (const float4*)((const float*)p + off);
+        const float4 a = src[0], b = src[1];
+        const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        split8(v, hi, lo);
+    } else {
+        hi = *(const bf16x8*)((const __bf16*)p + off);
+        lo = *(const bf16x8*)((const __bf16*)plo + off);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gemm_mfma_sk(const GemmJob* jobs, int nj, const DesignDev* dd) {
+    __shared__ float red[4][64][65];
+    const int j = find_job(jobs, nj, blockIdx.x);
+    const GemmJob& J = jobs[j];
+    const DesignDev& D = dd[J.design];
+    if (D.st->done) return;
+    const int t = blockIdx.x - J.tile_start;
+    const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const void* pa = J.a_kind ? J.A_hi : J.A;
+    const void* pb = J.b_kind ? J.B_hi : J.B;
+    f32x16 acc[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) acc[s][q] = 0.0f;
+    const int kq = J.K / 4;
+    const int kbeg = wv * kq, kend = kbeg + kq;
+    const int r = lane & 31, kh = (lane >> 5) * 8;
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
+        bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            frag_direct(J.a_kind, pa, J.A_lo, m0 + 32 * i + r, k0 + kh, J.K, ah[i], al[i]);
+            frag_direct(J.b_kind, pb, J.B_lo, n0 + 32 * i + r, k0 + kh, J.K, bh[i], bl[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int jj = 0; jj < 2; jj++) {
+                f32x16& c = acc[2 * i + jj];
+                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[jj], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[jj], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[jj], c, 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int row = 32 * i + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                const int col = 32 * jj + (lane & 31);
+                red[wv][row][col] = acc[2 * i + jj][q];
+            }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int row = e >> 6, col = e & 63;
+        const float v = ((red[0][row][col] + red[1][row][col]) + red[2][row][col]) + red[3][row][col];
+        epi_store<float>(J, D, m0 + row, n0 + col, (double)v);
+    }
+}
+
 void launch_gemm_pass(const GemmJob* jobs, int n_jobs, int total_tiles, const DesignDev* dd,
                       int prec, int engine, hipStream_t s) {
     if (n_jobs <= 0 || total_tiles <= 0) return;
@@ -223,6 +299,8 @@ void launch_gemm_pass(const GemmJob* jobs, int n_jobs, int total_tiles, const De
         hipLaunchKernelGGL(k_gemm_fma<double>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
     else if (engine == 0)
         hipLaunchKernelGGL(k_gemm_mfma, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+    else if (engine == 2)
+        hipLaunchKernelGGL(k_gemm_mfma_sk, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
     else
         hipLaunchKernelGGL(k_gemm_fma<float>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
 }

@@ -85,6 +85,9 @@ struct DesignDev {
     uint32_t* cs_idx;          // slots of each movable cell, ascending
     uint32_t* perm[2];         // cells in Morton order of their lower-left bin tile; the
                                // sort of iteration k reads perm[k&1] and writes perm[(k+1)&1]
+    uint32_t* keys;            // tile key of perm[k&1][j], by position j
+    void* sorted_xy;           // T [n_mov][2]: look-ahead positions in the new sort order
+    float* sorted_hs;          // half sides in the new sort order
     uint32_t* tcount;          // [n_keys] cells per tile key (zero between sorts)
     uint32_t* tstart;          // [n_keys + 1] first position of each key in perm
     uint32_t* tcur;            // [n_keys] fill cursors

@@ -423,7 +423,8 @@ __device__ inline uint32_t wave_agg_add(uint32_t* ctr, uint32_t key, bool valid)
 }
 
 // Counting sort of the cells by tile key, every iteration: count, scan, fill.
-// Cells are visited in the previous order, so a wave sees few distinct keys.
+// Both passes walk the cells in the previous iteration's order, in which
+// neighbours mostly share a tile, so a wave sees few distinct keys.
 template <typename T>
 __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
     const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
@@ -431,8 +432,11 @@ __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
     if (D.st->done) return;
     const int k = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
     const bool valid = k < D.n_mov;
-    const uint32_t* src = D.perm[D.st->it & 1];
-    const uint32_t key = valid ? tile_key<T>(D, (int)src[k]) : 0u;
+    uint32_t key = 0u;
+    if (valid) {
+        key = tile_key<T>(D, (int)D.perm[D.st->it & 1][k]);
+        D.keys[k] = key;
+    }
     wave_agg_add(D.tcount, key, valid);
 }
 
@@ -478,66 +482,85 @@ __global__ __launch_bounds__(256) void k_tile_fill(BatchDev b) {
     const bool valid = k < D.n_mov;
     const int par = D.st->it & 1;
     const uint32_t i = valid ? D.perm[par][k] : 0u;
-    const uint32_t key = valid ? tile_key<T>(D, (int)i) : 0u;
+    const uint32_t key = valid ? D.keys[k] : 0u;
     const uint32_t pos = wave_agg_add(D.tcur, key, valid);
-    if (valid && pos < (uint32_t)D.n_mov) D.perm[par ^ 1][pos] = i;
+    if (valid && pos < (uint32_t)D.n_mov) {
+        D.perm[par ^ 1][pos] = i;
+        T* sxy = (T*)D.sorted_xy;
+        sxy[2 * (size_t)pos] = ((const T*)D.look[0])[i];
+        sxy[2 * (size_t)pos + 1] = ((const T*)D.look[1])[i];
+        D.sorted_hs[pos] = D.hs[i];
+    }
 }
 
-// Pull scatter: one block per 8x8-bin tile.  Every cell whose footprint
-// reaches the tile has its lower-left bin in this tile or the tile to the
-// left, below, or below-left (footprints span at most 9 bins), so the block
-// reads those four key ranges of the sorted order, sums the overlaps that
-// fall inside its own 64 bins in LDS (fixed point on the FAST path, hence
-// independent of order), and writes the density and its overflow partials
-// with plain stores: no global atomics.
+// Pull scatter: one block per 32x32-bin super-tile (4x4 sort tiles, one
+// contiguous Morton key range).  Every cell whose footprint reaches the
+// super-tile has its lower-left bin in it or in the one-tile halo to its
+// left and below (footprints span at most 9 bins), so the block reads the
+// super-tile's key range plus the 9 halo tiles of the sorted copies, sums
+// the overlaps that fall inside its own 1,024 bins in LDS (fixed point on the
+// FAST path, hence independent of order), and writes the density and its
+// overflow partials with plain stores: no global atomics.
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
     typedef typename std::conditional<FIX, unsigned long long, double>::type Acc;
-    __shared__ Acc win[64];
+    __shared__ Acc win[32 * 32];
     __shared__ double sh[4 * kBinParts];
     const int d = find_blk(b.pfx_tile, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
     const int t = blockIdx.x - b.pfx_tile[d];
-    const int tx = t % D.tiles_x, ty = t / D.tiles_x;
-    const int x0 = tx * 8, y0 = ty * 8;
-    if (threadIdx.x < 64) win[threadIdx.x] = Acc(0);
+    const int stx = (D.tiles_x + 3) / 4;
+    const int sx = t % stx, sy = t / stx;
+    const int x0 = sx * 32, y0 = sy * 32;
+    for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) win[k] = Acc(0);
     __syncthreads();
-    const uint32_t* PERM = D.perm[(D.st->it + 1) & 1];
-    const T* LX = (const T*)D.look[0];
-    const T* LY = (const T*)D.look[1];
-    for (int q = 0; q < 4; q++) {
-        const int sx = tx - (q & 1), sy = ty - (q >> 1);
-        if (sx < 0 || sy < 0) continue;
-        const uint32_t key = morton2((uint32_t)sx, (uint32_t)sy);
-        if (key >= (uint32_t)D.n_keys) continue;
-        const uint32_t s0 = D.tstart[key], s1 = D.tstart[key + 1];
-        for (uint32_t k = s0 + threadIdx.x; k < s1; k += blockDim.x) {
-            const int i = PERM[k];
-            const BinBox r = bin_box(D, (double)LX[i], (double)LY[i], (double)D.hs[i]);
-            const int bx0 = max(r.bx0, x0), bx1 = min(r.bx1, x0 + 8);
-            const int by0 = max(r.by0, y0), by1 = min(r.by1, y0 + 8);
+    const T* SXY = (const T*)D.sorted_xy;
+    // ranges: q = 0 the 4x4 block of tiles (Morton keys 16*m .. 16*m+15),
+    // q = 1..9 the halo tiles (left column and bottom row, corner included)
+    for (int q = 0; q < 10; q++) {
+        uint32_t k0, k1;
+        if (q == 0) {
+            const uint32_t m = morton2((uint32_t)sx, (uint32_t)sy) * 16u;
+            if (m >= (uint32_t)D.n_keys) continue;
+            k0 = D.tstart[m];
+            k1 = D.tstart[min(m + 16u, (uint32_t)D.n_keys)];
+        } else {
+            int hx, hy;
+            if (q <= 5) { hx = sx * 4 - 1; hy = sy * 4 - 1 + (q - 1); }  // left column, from the corner
+            else { hx = sx * 4 + (q - 6); hy = sy * 4 - 1; }            // bottom row
+            if (hx < 0 || hy < 0) continue;
+            const uint32_t key = morton2((uint32_t)hx, (uint32_t)hy);
+            if (key >= (uint32_t)D.n_keys) continue;
+            k0 = D.tstart[key];
+            k1 = D.tstart[key + 1];
+        }
+        for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+            const BinBox r = bin_box(D, (double)SXY[2 * (size_t)k], (double)SXY[2 * (size_t)k + 1],
+                                     (double)D.sorted_hs[k]);
+            const int bx0 = max(r.bx0, x0), bx1 = min(r.bx1, x0 + 32);
+            const int by0 = max(r.by0, y0), by1 = min(r.by1, y0 + 32);
             for (int by = by0; by < by1; by++)
                 for (int bx = bx0; bx < bx1; bx++) {
                     const double w = ov_area(r, bx, by);
                     Acc v;
                     if constexpr (FIX) v = (Acc)__double2ll_rn(w * kFixScale);
                     else v = w;
-                    atomicAdd(&win[(by - y0) * 8 + (bx - x0)], v);
+                    atomicAdd(&win[(by - y0) * 32 + (bx - x0)], v);
                 }
         }
     }
     __syncthreads();
     double p[kBinParts] = {0.0, 0.0};
-    if (threadIdx.x < 64) {
-        const int bx = x0 + (threadIdx.x & 7), by = y0 + (threadIdx.x >> 3);
+    for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) {
+        const int bx = x0 + (k & 31), by = y0 + (k >> 5);
         if (bx < D.Bx && by < D.By) {
             double v;
-            if constexpr (FIX) v = (double)(long long)win[threadIdx.x] * (1.0 / kFixScale);
-            else v = win[threadIdx.x];
+            if constexpr (FIX) v = (double)(long long)win[k] * (1.0 / kFixScale);
+            else v = win[k];
             ((T*)D.dens)[(size_t)by * D.Bx + bx] = (T)v;
-            p[0] = v;
-            p[1] = v > 1.0 ? v - 1.0 : 0.0;
+            p[0] += v;
+            p[1] += v > 1.0 ? v - 1.0 : 0.0;
         }
     }
     block_sum<kBinParts>(p, sh);

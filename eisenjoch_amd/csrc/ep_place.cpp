@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -51,6 +52,17 @@ int validate_config(Ctx* c, const ep_config* cfg, int log_cap) {
     return EP_OK;
 }
 
+// Transform engine of a design shape: the autotuned choice when there is
+// one, else MFMA with LDS staging for BF16-eligible shapes (bins multiples of
+// 64 on the FAST path) and the FMA kernel otherwise.
+int engine_for(Ctx* c, int prec, int Bx, int By) {
+    const bool mfma_ok = prec == EP_PREC_FAST && Bx % 64 == 0 && By % 64 == 0 &&
+                         getenv("EP_FORCE_FMA") == nullptr;
+    if (!mfma_ok) return 1;
+    auto it = c->tune.find({Bx, By});
+    return it == c->tune.end() ? 0 : it->second;
+}
+
 // Per pass: (A kind, A matrix or buffer, B kind, B matrix or buffer, epilogue, out, out2)
 int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<const ep_config*>& cfgs,
                      Batch& b) {
@@ -68,7 +80,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     std::vector<int32_t> pc(nd + 1), pb(nd + 1), pw(nd + 1), ph(nd + 1), ps(nd + 1), pt(nd + 1);
     pc[0] = pb[0] = pw[0] = ph[0] = ps[0] = pt[0] = 0;
     // jobs: [pass][engine]
-    std::vector<GemmJob> jobs[4][2];
+    std::vector<GemmJob> jobs[4][kEngines];
     for (int i = 0; i < nd; i++) {
         Design& D = *ds[i];
         dv[i] = D.dev;
@@ -113,9 +125,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         cst(i2a, true, ty, 1); dat(i2a, false, D.dev.uT[0]);
         GemmJob i2b = mk(By, Bx, By, kEpiStore, D.dev.field[1], nullptr);  // S3 . U (idst_cols)
         cst(i2b, true, ty, 2); dat(i2b, false, D.dev.uT[1]);
-        const bool mfma_ok = b.prec == EP_PREC_FAST && Bx % 64 == 0 && By % 64 == 0 &&
-                             getenv("EP_FORCE_FMA") == nullptr;
-        const int eng = mfma_ok ? 0 : 1;
+        const int eng = engine_for(c, b.prec, Bx, By);
         jobs[0][eng].push_back(f1);
         jobs[1][eng].push_back(f2);
         jobs[2][eng].push_back(i1a);
@@ -132,7 +142,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     // device image: descriptors, prefixes, jobs
     size_t bytes = align_up(sizeof(DesignDev) * nd) + 6 * align_up(sizeof(int32_t) * (nd + 1)) + align_up(4);
     for (int p = 0; p < 4; p++)
-        for (int e = 0; e < 2; e++) {
+        for (int e = 0; e < kEngines; e++) {
             int t0 = 0;
             for (auto& J : jobs[p][e]) {
                 J.tile_start = t0;
@@ -163,7 +173,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     const int32_t zero = 0;
     b.dzero = (const int32_t*)put(&zero, 4);
     for (int p = 0; p < 4; p++)
-        for (int e = 0; e < 2; e++) {
+        for (int e = 0; e < kEngines; e++) {
             GemmJob dummy;
             memset(&dummy, 0, sizeof(dummy));
             if (jobs[p][e].empty()) b.jobs2[p][e] = (GemmJob*)put(&dummy, sizeof(GemmJob));
@@ -234,7 +244,7 @@ int batch_reset(Ctx* c, Batch& b, const ep_config* cfg) {
 }
 
 static void enqueue_gemm(Ctx* c, Batch& b, int p) {
-    for (int e = 0; e < 2; e++)
+    for (int e = 0; e < kEngines; e++)
         if (b.njobs2[p][e] > 0)
             launch_gemm_pass(b.jobs2[p][e], b.njobs2[p][e], b.tiles2[p][e], b.d_dev, b.prec, e, c->stream);
 }
@@ -487,7 +497,135 @@ int ep_iter_log(ep_ctx* h, uint32_t id, ep_iter_rec* recs, uint32_t cap, uint32_
 
 }  // extern "C"
 
+// ---------------------------------------------------------- autotuner ----
+
+// Times every eligible transform engine on one resident design of each
+// distinct bin shape among `ids` and keeps the fastest in the context's
+// table.  With a cache path, shapes already in the file are taken from it
+// and new results are appended.  Engines differ only in FP32 rounding order,
+// and the table is per shape, so a design's results do not depend on the
+// batch it runs in.
+extern "C" int ep_autotune(ep_ctx* h, const uint32_t* ids, uint32_t n, uint32_t reps,
+                           const char* cache_path) {
+    Ctx* c = (Ctx*)h;
+    if (!c || (!ids && n)) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    if (c->server_running) return fail(c, EP_ERR_INVALID_ARG, "context is serving; stop it first");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::map<std::pair<int, int>, int> cached;
+    if (cache_path) {
+        FILE* f = fopen(cache_path, "r");
+        if (f) {
+            int bx, by, e;
+            while (fscanf(f, "%d %d %d", &bx, &by, &e) == 3)
+                if (e >= 0 && e < kEngines) cached[{bx, by}] = e;
+            fclose(f);
+        }
+    }
+    if (reps == 0) reps = 5;
+    std::map<std::pair<int, int>, Design*> shapes;
+    for (uint32_t i = 0; i < n; i++) {
+        Design* D = find_design(c, ids[i]);
+        if (!D) return fail(c, EP_ERR_NOT_FOUND, "unknown design id");
+        if (D->prec != EP_PREC_FAST || D->dev.Bx % 64 || D->dev.By % 64) continue;
+        shapes.emplace(std::make_pair(D->dev.Bx, D->dev.By), D);
+    }
+    ep_config cfg;
+    ep_config_default(&cfg);
+    cfg.field_mode = EP_FIELD_IDXST;
+    std::vector<std::pair<std::pair<int, int>, int>> fresh;
+    for (auto& kv : shapes) {
+        auto hit = cached.find(kv.first);
+        if (hit != cached.end()) {
+            c->tune[kv.first] = hit->second;
+            continue;
+        }
+        double best = 1e300;
+        int best_e = 0;
+        for (int e : {0, 2, 1}) {
+            c->tune[kv.first] = e;
+            release_batch_cache(c);
+            Batch b;
+            std::vector<Design*> ds{kv.second};
+            std::vector<const ep_config*> cf{&cfg};
+            int rc = batch_build_cfgs(c, ds, cf, b);
+            if (rc) return rc;
+            // a live design state so the transform kernels do not skip it
+            DevState st;
+            memset(&st, 0, sizeof(st));
+            st.a = 1.0;
+            c->state_host[kv.second->state_idx] = st;
+            hipError_t er = hipMemcpyAsync(kv.second->dev.st, &c->state_host[kv.second->state_idx],
+                                           sizeof(DevState), hipMemcpyHostToDevice, c->stream);
+            hipEvent_t e0, e1;
+            if (er == hipSuccess) er = hipEventCreate(&e0);
+            if (er == hipSuccess) er = hipEventCreate(&e1);
+            if (er != hipSuccess) { batch_release(b); return fail(c, EP_ERR_DEVICE, hipGetErrorString(er)); }
+            for (int p = 0; p < 4; p++) enqueue_gemm(c, b, p);  // warm
+            hipEventRecord(e0, c->stream);
+            for (uint32_t r = 0; r < reps; r++)
+                for (int p = 0; p < 4; p++) enqueue_gemm(c, b, p);
+            hipEventRecord(e1, c->stream);
+            er = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (er == hipSuccess) er = hipEventElapsedTime(&ms, e0, e1);
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            batch_release(b);
+            if (er != hipSuccess) return fail(c, EP_ERR_DEVICE, hipGetErrorString(er));
+            if (ms < best) { best = ms; best_e = e; }
+        }
+        c->tune[kv.first] = best_e;
+        fresh.push_back({kv.first, best_e});
+    }
+    release_batch_cache(c);
+    if (cache_path && !fresh.empty()) {
+        FILE* f = fopen(cache_path, "a");
+        if (f) {
+            for (auto& r : fresh) fprintf(f, "%d %d %d\n", r.first.first, r.first.second, r.second);
+            fclose(f);
+        }
+    }
+    return EP_OK;
+}
+
+// Reads the tuning table: up to cap (Bx, By, engine) triples.
+extern "C" int ep_tune_table(ep_ctx* h, int32_t* out, uint32_t cap, uint32_t* n) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !n || (!out && cap)) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    uint32_t k = 0;
+    for (auto& kv : c->tune) {
+        if (k < cap) {
+            out[3 * k] = kv.first.first;
+            out[3 * k + 1] = kv.first.second;
+            out[3 * k + 2] = kv.second;
+        }
+        k++;
+    }
+    *n = k;
+    return EP_OK;
+}
+
 // ---------------------------------------------------------- profiling ----
+
+extern "C" int ep_calibrate_copy(ep_ctx* h, uint64_t bytes) {
+    Ctx* c = (Ctx*)h;
+    if (!c || bytes == 0) return fail(c, EP_ERR_INVALID_ARG, "bad argument");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    HIPCHK(c, hipSetDevice(c->device));
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 1, bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(a);
+    if (b) hipFree(b);
+    if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, hipGetErrorString(e));
+    return EP_OK;
+}
 
 extern "C" int ep_profile_iteration(ep_ctx* h, const uint32_t* ids, uint32_t n, const ep_config* cfg,
                                     uint32_t warm, uint32_t reps, ep_phase_time* out, uint32_t cap,
@@ -690,7 +828,7 @@ int ep_eval_field(ep_ctx* h, uint32_t id, const double* dens, double* ex, double
     const size_t NB = (size_t)D->dev.Bx * D->dev.By;
     if ((rc = put_T(c, D->dev.dens, dens, NB, D->prec))) return rc;
     for (int p = 0; p < 4; p++)
-        for (int e = 0; e < 2; e++)
+        for (int e = 0; e < kEngines; e++)
             if (b->njobs2[p][e] > 0)
                 launch_gemm_pass(b->jobs2[p][e], b->njobs2[p][e], b->tiles2[p][e], b->d_dev, b->prec, e, c->stream);
     HIPCHK(c, hipGetLastError());

@@ -265,6 +265,9 @@ static size_t plan_arena(Design& D, char* base) {
     G.cs_idx = (uint32_t*)P(D.cs_count * 4);
     G.perm[0] = (uint32_t*)P(N * 4);
     G.perm[1] = (uint32_t*)P(N * 4);
+    G.keys = (uint32_t*)P(N * 4);
+    G.sorted_xy = P(N * 2 * T);
+    G.sorted_hs = (float*)P(N * 4);
     G.tcount = (uint32_t*)P((size_t)G.n_keys * 4);
     G.tstart = (uint32_t*)P(((size_t)G.n_keys + 1) * 4);
     G.tcur = (uint32_t*)P((size_t)G.n_keys * 4);
@@ -347,7 +350,7 @@ static void fill_desc_sizes(Design& D, const ep_design* d, const NetLayout& L) {
                 hs_max = std::max(hs_max, 0.5 * std::sqrt((double)d->area[i]));
         G.scat_pull = (2.0 * hs_max * std::max(G.rx, G.ry) <= 6.5) ? 1 : 0;
         if (G.scat_pull) {
-            G.n_tile_blocks = G.tiles_x * G.tiles_y;
+            G.n_tile_blocks = ((G.tiles_x + 3) / 4) * ((G.tiles_y + 3) / 4);
             G.n_ov_parts = G.n_tile_blocks;
             G.n_scat_blocks = 0;
             G.n_bin_blocks = 0;
@@ -356,7 +359,7 @@ static void fill_desc_sizes(Design& D, const ep_design* d, const NetLayout& L) {
             G.n_ov_parts = G.n_bin_blocks;
         }
     }
-    D.cs_count = L.cs_idx.size();
+    D.cs_count = L.cs_off.empty() ? 0 : L.cs_off.back();
 }
 
 }  // namespace ep

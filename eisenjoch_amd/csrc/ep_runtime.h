@@ -18,6 +18,8 @@ struct ep_ctx {};
 namespace ep {
 
 constexpr int kStatePool = 8192;      // resident designs per context
+// Transform engines: 0 MFMA with LDS staging, 1 FP32/FP64 FMA, 2 MFMA split-K.
+constexpr int kEngines = 3;
 constexpr int kDefaultLogCap = 2048;  // iteration records per design
 
 // Constant transform matrices of one size and field mode: [0] C2, [1] C3,
@@ -44,8 +46,9 @@ struct Design {
 };
 
 // A batch of resident designs sharing every launch of an iteration.  The
-// transform jobs of each pass are split by engine ([pass][0] MFMA, [pass][1]
-// FMA) per design shape, so a design takes the same path in any batch.
+// transform jobs of each pass are split by engine ([pass][engine]); a
+// design's engine depends only on its own shape (and the context's tuning
+// table), so it takes the same path in any batch.
 struct Batch {
     std::vector<Design*> ds;
     std::vector<ep_config> cfgs;
@@ -53,9 +56,9 @@ struct Batch {
     int log_hpwl = 1;
     int cell_blocks = 0, bin_blocks = 0, wa_blocks = 0, hf_blocks = 0, scat_blocks = 0;
     int tile_blocks = 0;
-    int njobs2[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    int tiles2[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    GemmJob* jobs2[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    int njobs2[4][kEngines] = {};
+    int tiles2[4][kEngines] = {};
+    GemmJob* jobs2[4][kEngines] = {};
     void* mem = nullptr;        // descriptors, prefixes, jobs
     DesignDev* d_dev = nullptr;
     const int32_t* dzero = nullptr;  // a device zero: prefix of a one-design view
@@ -91,6 +94,7 @@ struct Ctx : ep_ctx {
     bool cache_cfg_valid = false;
     ep_config eval_cfg{};            // configuration of the ep_eval_* calls
     bool server_running = false;
+    std::map<std::pair<int, int>, int> tune;   // (Bx, By) -> transform engine (autotuned)
     std::unique_ptr<Server, ServerDeleter> server;
 };
 

@@ -400,6 +400,21 @@ class Device:
                                             C.byref(k)), self._h, "ep_profile_iteration")
         return {out[i].name.decode(): out[i].avg_ms for i in range(k.value)}
 
+    def autotune(self, ids: Sequence[int], reps: int = 5, cache_path: Optional[str] = None) -> dict:
+        """Pick the fastest transform engine per bin shape (ep_autotune)."""
+        n = len(ids)
+        arr = (C.c_uint32 * n)(*ids)
+        path = cache_path.encode() if cache_path else None
+        _check(L.lib().ep_autotune(self._h, arr, n, reps, path), self._h, "ep_autotune")
+        out = (C.c_int32 * 3 * 64)()
+        k = C.c_uint32()
+        _check(L.lib().ep_tune_table(self._h, C.cast(out, C.POINTER(C.c_int32)), 64, C.byref(k)),
+               self._h, "ep_tune_table")
+        names = {0: "mfma-staged", 1: "fma", 2: "mfma-splitk"}
+        flat = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_int32)), shape=(64 * 3,))
+        return {f"{flat[3 * i]}x{flat[3 * i + 1]}": names.get(int(flat[3 * i + 2]), "?")
+                for i in range(min(k.value, 64))}
+
     # -- single-kernel evaluation --
     def eval_config(self, cfg: ElectroPlaceCfg):
         _check(L.lib().ep_eval_config(self._h, C.byref(cfg.to_c())), self._h, "ep_eval_config")

@@ -197,6 +197,20 @@ int ep_profile_iteration(ep_ctx* ctx, const uint32_t* ids, uint32_t n, const ep_
                          uint32_t warm, uint32_t reps, ep_phase_time* out, uint32_t cap,
                          uint32_t* n_out);
 
+/* Autotuner (no counterpart in the reference): times every eligible
+ * transform engine (MFMA with LDS staging, MFMA split-K, FP32 FMA) on one
+ * resident FAST design of each distinct bin shape among `ids`, `reps` times,
+ * and keeps the fastest per shape for every later batch.  With a cache path,
+ * shapes found in the file are reused and new results appended. */
+int ep_autotune(ep_ctx* ctx, const uint32_t* ids, uint32_t n, uint32_t reps, const char* cache_path);
+/* The tuning table as (Bx, By, engine) triples; *n receives the entry count. */
+int ep_tune_table(ep_ctx* ctx, int32_t* out, uint32_t cap, uint32_t* n);
+
+/* Copies `bytes` device-to-device once with the runtime's plain copy kernel
+ * (no counterpart in the reference): a transfer of known size against which
+ * hardware traffic counters are calibrated. */
+int ep_calibrate_copy(ep_ctx* ctx, uint64_t bytes);
+
 /* ---- host helpers ---- */
 /* Greedy legalization, mod.rs:206-256: cells in index order take the free
  * BEL of their bucket nearest in squared distance, first minimum in BEL
