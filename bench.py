@@ -53,7 +53,71 @@ def parse():
                    help="serving scenario: Poisson arrival rate in requests/s (0: batch workload)")
     p.add_argument("--duration", type=float, default=120.0, help="serving: arrival window in seconds")
     p.add_argument("--pool", type=int, default=96, help="serving: distinct pre-generated designs")
+    p.add_argument("--workload", default="batch", choices=["batch", "config2", "config3"],
+                   help="batch (default), or one large design: config2 (200k cells, 512 bins) "
+                        "or config3 (1M cells, high-fanout nets, 1,024 bins)")
     return p.parse_args()
+
+
+def single(args):
+    """One large design: the plain FP64 GPU placer against the FAST one
+    (FP32 bulk, BF16 hi+lo MFMA transforms, FP64 reductions), same seeded
+    netlist and initial positions; for config 3 also the FAST field's error
+    against the FP64 field on the same density."""
+    import eisenjoch_amd as ep
+    d = ep.config2() if args.workload == "config2" else ep.config3()
+    cfg = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
+    dev = ep.Device(0)
+    ids = {p: dev.upload(d, p) for p in ("fp64", "fast")}
+    res = {}
+    for p, i in ids.items():
+        dev.place(i, cfg, positions=False)  # warm (transforms, graph)
+        t0 = time.perf_counter()
+        reps = max(1, args.steps)
+        for _ in range(reps):
+            r = dev.place(i, cfg, positions=True)
+        dt = (time.perf_counter() - t0) / reps
+        res[p] = (r, dt)
+    r64, t64 = res["fp64"]
+    r32, t32 = res["fast"]
+    out = {
+        "metric": f"single design ({args.workload}): time to overflow<=0.10, FAST vs plain FP64",
+        "value": round(1.0 / t32, 3), "unit": "designs/s", "n_gpus": 1, "steps": max(1, args.steps),
+        "higher_is_better": True,
+        "config": {"workload": d.name, "cells": d.n_mov, "nets": d.n_nets, "pins": d.n_pins,
+                   "bins": [d.Bx, d.By], "tiles": [d.W, d.H]},
+        "fast": {"seconds": round(t32, 4), "iters": r32.iters, "converged": r32.converged,
+                 "overflow": r32.overflow, "hpwl": r32.hpwl, "ms_per_iter": round(t32 / r32.iters * 1e3, 4)},
+        "plain_fp64": {"seconds": round(t64, 4), "iters": r64.iters, "converged": r64.converged,
+                       "overflow": r64.overflow, "hpwl": r64.hpwl,
+                       "ms_per_iter": round(t64 / r64.iters * 1e3, 4)},
+        "speedup_fast_vs_fp64": round(t64 / t32, 2),
+        "hpwl_rel_dev": abs(r32.hpwl - r64.hpwl) / r64.hpwl,
+    }
+    if args.workload == "config3":
+        dens, _ = dev.density(ids["fp64"], r32.x, r32.y)
+        dev.eval_config(ep.ElectroPlaceCfg(field_mode="idxst"))
+        e64 = dev.field(ids["fp64"], dens)
+        e32 = dev.field(ids["fast"], dens)
+        err = {}
+        for k, (a, b) in enumerate(zip(e32, e64)):
+            diff = np.abs(a - b)
+            err["ex" if k == 0 else "ey"] = {
+                "max_abs_over_max": float(diff.max() / np.abs(b).max()),
+                "rms_rel": float(np.sqrt((diff ** 2).mean() / (b ** 2).mean()))}
+        out["field_error_vs_fp64"] = err
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import _oracle as O
+        it = 3
+        t0 = time.perf_counter()
+        O.place(d, max_iters=it, field_mode=1)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(dt / it * 1e3, 2), "unit": "ms/iteration", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{it} iterations of the FP64 C oracle on the same design, one core"}
+    print(json.dumps(out), flush=True)
+    dev.close()
 
 
 def serve(args):
@@ -162,6 +226,9 @@ def main():
     args = parse()
     if args.serve > 0 and args.gpus == 1:
         serve(args)
+        return
+    if args.workload != "batch" and args.gpus == 1:
+        single(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this placer runs on one GPU; designs shard across GPUs "
