@@ -45,7 +45,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU oracle sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
-    p.add_argument("--profile-only", action="store_true", help="one step, no baselines (for rocprof)")
+    p.add_argument("--profile-only", action="store_true",
+                   help="only the per-phase timing of the batched iteration (8 graph-replayed "
+                        "warm iterations, then 5 timed ones, 64 designs active): for rocprof")
     p.add_argument("--calib-copy", type=int, default=0,
                    help="profile mode: also copy this many MiB device-to-device with one plain "
                         "copy kernel (counter calibration)")
@@ -53,19 +55,23 @@ def parse():
                    help="serving scenario: Poisson arrival rate in requests/s (0: batch workload)")
     p.add_argument("--duration", type=float, default=120.0, help="serving: arrival window in seconds")
     p.add_argument("--pool", type=int, default=96, help="serving: distinct pre-generated designs")
-    p.add_argument("--workload", default="batch", choices=["batch", "config2", "config3"],
-                   help="batch (default), or one large design: config2 (200k cells, 512 bins) "
-                        "or config3 (1M cells, high-fanout nets, 1,024 bins)")
+    p.add_argument("--workload", default="batch", choices=["batch", "config1", "config2", "config3"],
+                   help="batch (default), or one design: config1 (1,500 cells, 64x64, the "
+                        "reference's own size), config2 (200k cells, 512 bins) or config3 "
+                        "(1M cells, high-fanout nets, 1,024 bins)")
+    p.add_argument("--traffic-file", default=str(ROOT / "profiles" / "pmc_traffic_batched64.json"),
+                   help="per-dispatch FETCH_SIZE/WRITE_SIZE of the batched iteration's kernels, "
+                        "from rocprofv3 --pmc runs of --profile-only (reported as roofline.traffic)")
     return p.parse_args()
 
 
 def single(args):
-    """One large design: the plain FP64 GPU placer against the FAST one
+    """One design: the plain FP64 GPU placer against the FAST one
     (FP32 bulk, BF16 hi+lo MFMA transforms, FP64 reductions), same seeded
     netlist and initial positions; for config 3 also the FAST field's error
     against the FP64 field on the same density."""
     import eisenjoch_amd as ep
-    d = ep.config2() if args.workload == "config2" else ep.config3()
+    d = {"config1": ep.config1, "config2": ep.config2, "config3": ep.config3}[args.workload]()
     cfg = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
     dev = ep.Device(0)
     ids = {p: dev.upload(d, p) for p in ("fp64", "fast")}
@@ -106,18 +112,51 @@ def single(args):
                 "max_abs_over_max": float(diff.max() / np.abs(b).max()),
                 "rms_rel": float(np.sqrt((diff ** 2).mean() / (b ** 2).mean()))}
         out["field_error_vs_fp64"] = err
+    if args.workload == "config1":
+        # the host legalizer (ring search, reference tie rule) on the FAST result
+        t0 = time.perf_counter()
+        cell_bel, _ = ep.legalize(r32.x, r32.y, d.cell_type, d.bel_x, d.bel_y, d.bel_type)
+        out["fast"]["legalize_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        out["fast"]["legalized_hpwl"] = ep.total_hpwl_int(d, d.bel_x[cell_bel], d.bel_y[cell_bel])
+    dev.close()
     if not args.no_cpu:
+        host = pin_one_core()
         sys.path.insert(0, str(ROOT / "tests"))
         import _oracle as O
-        it = 3
-        t0 = time.perf_counter()
-        O.place(d, max_iters=it, field_mode=1)
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(dt / it * 1e3, 2), "unit": "ms/iteration", "cores": 1,
-                               "kind": "port",
-                               "sample": f"{it} iterations of the FP64 C oracle on the same design, one core"}
+        if args.workload == "config1":
+            # whole continuous loop to the stop rule, then the reference's
+            # legalization (full bucket scan) and integer HPWL once per
+            # 5-iteration interval, as its loop runs them
+            t0 = time.perf_counter()
+            o = O.place(d, max_iters=500, field_mode=1)
+            t_loop = time.perf_counter() - t0
+            n_leg = (o["iters"] + 4) // 5
+            t0 = time.perf_counter()
+            for _ in range(n_leg):
+                rc, cb, _ = O.legalize(o["x"], o["y"], d.cell_type, d.bel_x, d.bel_y, d.bel_type)
+                if rc != 0:
+                    raise RuntimeError(f"oracle legalization failed: {rc}")
+                ep.total_hpwl_int(d, d.bel_x[cb], d.bel_y[cb])
+            t_leg = time.perf_counter() - t0
+            out["cpu_baseline"] = {
+                "value": round(t_loop * 1e3, 2), "unit": "ms to overflow<=0.10 (continuous loop)",
+                "cores": 1, "kind": "port",
+                "ms_per_iteration": round(t_loop / o["iters"] * 1e3, 3), "iters": o["iters"],
+                "full_loop_ms": round((t_loop + t_leg) * 1e3, 2),
+                "legalizations": n_leg,
+                "sample": "the whole design with the FP64 C oracle on one core; full_loop_ms adds the "
+                          "reference's legalization (full bucket scan) and integer HPWL once per "
+                          "5 iterations", **host}
+        else:
+            it = 3
+            t0 = time.perf_counter()
+            O.place(d, max_iters=it, field_mode=1)
+            dt = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(dt / it * 1e3, 2), "unit": "ms/iteration", "cores": 1,
+                                   "kind": "port",
+                                   "sample": f"{it} iterations of the FP64 C oracle on the same design, "
+                                             f"one core", **host}
     print(json.dumps(out), flush=True)
-    dev.close()
 
 
 def serve(args):
@@ -202,9 +241,64 @@ def algorithmic_work(designs):
     return {"wa": wa, "scatter": scat, "gather+update": gath, "dct_flops": dct}
 
 
+# Kernel groups of one batched iteration: the profiler phases they span, the
+# kernels (rocprofv3 names) and launches per iteration behind them, and which
+# algorithmic figure of algorithmic_work() prices them.
+KERNEL_GROUPS = {
+    "gather+update": {"phases": ["gather", "update"], "bound": "hbm", "work": "gather+update",
+                      "kernels": {"ep::k_gather<float, true, 0>": 1, "ep::k_update<float, true>": 1}},
+    "scatter (tile sort + pull scatter)": {
+        "phases": ["sort", "scatter", "dens_final"], "bound": "hbm", "work": "scatter",
+        "kernels": {"ep::k_tile_count<float>": 1, "ep::k_tile_scan(": 1, "ep::k_tile_fill<float>": 1,
+                    "ep::k_scatter_pull<float, true>": 1}},
+    "wa": {"phases": ["wa"], "bound": "hbm", "work": "wa",
+           "kernels": {"ep::k_wa<float, true, false>": 1}},
+    "dct (4 transform passes, BF16 hi+lo MFMA)": {
+        "phases": ["dct_f1", "dct_f2", "dct_i1", "dct_i2"], "bound": "mfma", "work": "dct_flops",
+        "kernels": {"ep::k_gemm_mfma(": 4}},
+}
+
+
+def group_traffic(path, group):
+    """HBM bytes per iteration of a kernel group from the committed rocprofv3
+    --pmc summary (per-dispatch FETCH_SIZE / WRITE_SIZE in KiB, each scaled by
+    what the counter reads for a plain copy of known size); None without it."""
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    total = 0.0
+    for prefix, launches in group["kernels"].items():
+        hit = [v for k, v in t["kernels"].items() if prefix in k]
+        if not hit:
+            return None
+        total += launches * 1024.0 * (hit[0]["fetch_kib"] * t["fetch_factor"]
+                                      + hit[0]["write_kib"] * t["write_factor"])
+    return round(total)
+
+
+def pin_one_core():
+    """Pin this process to its first allowed core (the reference's loop is
+    sequential) once the GPU work is done; describe the host."""
+    allowed = sorted(os.sched_getaffinity(0))
+    os.sched_setaffinity(0, {allowed[0]})
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"core": allowed[0], "host_cores_allowed": len(allowed), "cpu_model": model}
+
+
 def cpu_baseline(designs, budget_s):
     """FP64 oracle (tests/_oracle.py, the checker) on a bounded sample of the
     batch, one core; returns designs/sec over the designs it finished."""
+    host = pin_one_core()
     sys.path.insert(0, str(ROOT / "tests"))
     import _oracle as O
     order = list(range(0, len(designs), 8)) + [i for i in range(len(designs)) if i % 8]
@@ -219,7 +313,7 @@ def cpu_baseline(designs, budget_s):
     return {"value": round(len(done) / dt, 4), "unit": "designs/s", "cores": 1, "kind": "port",
             "sample": f"{len(done)} of the {len(designs)} batch designs (indices {done}, "
                       f"{sum(designs[i].n_mov for i in done)} cells) placed to overflow <= 0.10 "
-                      f"by the FP64 C oracle in {dt:.1f} s on one host core"}
+                      f"by the FP64 C oracle in {dt:.1f} s on one host core", **host}
 
 
 def main():
@@ -242,9 +336,24 @@ def main():
     dev = ep.Device(0)
     ids = [dev.upload(d, "fast") for d in designs]
 
+    if args.profile_only:
+        prof = dev.profile(ids, cfg, warm=8, reps=5)
+        out = {"profile_ms_per_iter": {k: round(v, 4) for k, v in prof.items()},
+               "designs": len(ids), "warm": 8, "reps": 5}
+        if args.calib_copy:
+            from eisenjoch_amd import _lib
+            nbytes = args.calib_copy << 20
+            rc = _lib.lib().ep_calibrate_copy(dev.handle, nbytes)
+            if rc != 0:
+                raise RuntimeError(f"ep_calibrate_copy failed: {rc}")
+            out["calib_copy_bytes"] = nbytes
+        print(json.dumps(out), flush=True)
+        dev.close()
+        return
+
     for _ in range(args.warmup):
         dev.place_batch(ids, cfg, positions=False)
-    steps = 1 if args.profile_only else args.steps
+    steps = args.steps
     t0 = time.perf_counter()
     for _ in range(steps):
         res = dev.place_batch(ids, cfg, positions=False)
@@ -278,17 +387,6 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p95": float(np.percentile(lat, 95)),
                        "p99": float(np.percentile(lat, 99))},
     }
-    if args.profile_only:
-        if args.calib_copy:
-            from eisenjoch_amd import _lib
-            nbytes = args.calib_copy << 20
-            rc = _lib.lib().ep_calibrate_copy(dev.handle, nbytes)
-            if rc != 0:
-                raise RuntimeError(f"ep_calibrate_copy failed: {rc}")
-            out["calib_copy_bytes"] = nbytes
-        print(json.dumps(out), flush=True)
-        return
-
     # plain FP64 GPU placer, one design at a time (reference point, not timed above)
     if not args.no_plain:
         plain_ids = [dev.upload(d, "fp64") for d in designs]
@@ -306,30 +404,33 @@ def main():
         for i in plain_ids:
             dev.free(i)
 
-    # roofline of the dominant kernel, HIP events on the context stream
+    # roofline of the dominant kernel group, HIP events on the context stream
     prof = dev.profile(ids, cfg, warm=8, reps=5)
     work = algorithmic_work(designs)
-    dct_ms = sum(prof[k] for k in ("dct_f1", "dct_f2", "dct_i1", "dct_i2"))
-    hbm = {"wa": (prof["wa"], work["wa"]), "scatter": (prof["scatter"], work["scatter"]),
-           "gather+update": (prof["gather"] + prof["update"], work["gather+update"])}
-    cands = {k: v[0] for k, v in hbm.items()}
-    cands["dct"] = dct_ms
-    dom = max(cands, key=cands.get)
-    if dom == "dct":
-        ach = work["dct_flops"] / (dct_ms * 1e-3) / 1e12
-        out["roofline"] = {"kernel": "dct (4 transform passes, BF16 hi+lo MFMA)", "bound": "mfma",
-                           "achieved": round(ach, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                           "frac": round(ach / BF16_PEAK_TFLOPS, 5), "traffic": None}
+    groups = {name: (sum(prof[ph] for ph in g["phases"]), g) for name, g in KERNEL_GROUPS.items()}
+    dom = max(groups, key=lambda k: groups[k][0])
+    ms, g = groups[dom]
+    traffic = group_traffic(args.traffic_file, g)
+    if g["bound"] == "mfma":
+        ach = work[g["work"]] / (ms * 1e-3) / 1e12
+        out["roofline"] = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3),
+                           "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(ach / BF16_PEAK_TFLOPS, 5), "traffic": traffic}
     else:
-        ms, b = hbm[dom]
-        ach = b / (ms * 1e-3) / 1e9
+        ach = work[g["work"]] / (ms * 1e-3) / 1e9
         out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
-                           "traffic": None}
+                           "traffic": traffic}
+    out["roofline"]["algorithmic_per_launch"] = work[g["work"]]
+    out["roofline"]["launch_us"] = round(ms * 1e3, 2)
     out["phase_ms_per_iter"] = {k: round(v, 4) for k, v in prof.items()}
-    out["kernel_rooflines"] = {
-        k: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in hbm.items() if v[0] > 0}
-    out["kernel_rooflines"]["dct_tflops"] = round(work["dct_flops"] / (dct_ms * 1e-3) / 1e12, 2) if dct_ms else None
+    out["kernel_rooflines"] = {}
+    for name, (gms, gg) in groups.items():
+        if gms <= 0:
+            continue
+        scale, unit = (1e12, "TFLOP/s") if gg["bound"] == "mfma" else (1e9, "GB/s")
+        out["kernel_rooflines"][name] = {"us": round(gms * 1e3, 2), unit: round(
+            work[gg["work"]] / (gms * 1e-3) / scale, 2), "traffic": group_traffic(args.traffic_file, gg)}
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(designs, args.cpu_seconds)

@@ -86,6 +86,8 @@ struct DesignDev {
     uint32_t* perm[2];         // cells in Morton order of their lower-left bin tile; the
                                // sort of iteration k reads perm[k&1] and writes perm[(k+1)&1]
     uint32_t* keys;            // tile key of perm[k&1][j], by position j
+    void* stage_xy;            // T [n_mov][2]: look-ahead position of perm[k&1][j], by j
+    float* stage_hs;           // half side of perm[k&1][j], by j
     void* sorted_xy;           // T [n_mov][2]: look-ahead positions in the new sort order
     float* sorted_hs;          // half sides in the new sort order
     uint32_t* tcount;          // [n_keys] cells per tile key (zero between sorts)

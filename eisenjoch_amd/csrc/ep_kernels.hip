@@ -394,9 +394,8 @@ __device__ inline uint32_t morton2(uint32_t x, uint32_t y) {
 }
 
 template <typename T>
-__device__ inline uint32_t tile_key(const DesignDev& D, int i) {
-    const BinBox r = bin_box(D, (double)((const T*)D.look[0])[i], (double)((const T*)D.look[1])[i],
-                             (double)D.hs[i]);
+__device__ inline uint32_t tile_key(const DesignDev& D, T x, T y, float hs) {
+    const BinBox r = bin_box(D, (double)x, (double)y, (double)hs);
     const int bx = min(r.bx0, D.Bx - 1), by = min(r.by0, D.By - 1);
     const uint32_t k = morton2((uint32_t)bx >> 3, (uint32_t)by >> 3);
     return k < (uint32_t)D.n_keys ? k : (uint32_t)D.n_keys - 1;
@@ -424,7 +423,9 @@ __device__ inline uint32_t wave_agg_add(uint32_t* ctr, uint32_t key, bool valid)
 
 // Counting sort of the cells by tile key, every iteration: count, scan, fill.
 // Both passes walk the cells in the previous iteration's order, in which
-// neighbours mostly share a tile, so a wave sees few distinct keys.
+// neighbours mostly share a tile, so a wave sees few distinct keys.  The
+// count pass gathers each cell's position and half side once and stages
+// them by position, so the fill pass reads only contiguous arrays.
 template <typename T>
 __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
     const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
@@ -434,8 +435,15 @@ __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
     const bool valid = k < D.n_mov;
     uint32_t key = 0u;
     if (valid) {
-        key = tile_key<T>(D, (int)D.perm[D.st->it & 1][k]);
+        const uint32_t i = D.perm[D.st->it & 1][k];
+        const T x = ((const T*)D.look[0])[i], y = ((const T*)D.look[1])[i];
+        const float hs = D.hs[i];
+        key = tile_key<T>(D, x, y, hs);
         D.keys[k] = key;
+        T* sxy = (T*)D.stage_xy;
+        sxy[2 * (size_t)k] = x;
+        sxy[2 * (size_t)k + 1] = y;
+        D.stage_hs[k] = hs;
     }
     wave_agg_add(D.tcount, key, valid);
 }
@@ -485,11 +493,12 @@ __global__ __launch_bounds__(256) void k_tile_fill(BatchDev b) {
     const uint32_t key = valid ? D.keys[k] : 0u;
     const uint32_t pos = wave_agg_add(D.tcur, key, valid);
     if (valid && pos < (uint32_t)D.n_mov) {
+        const T* gxy = (const T*)D.stage_xy;
         D.perm[par ^ 1][pos] = i;
         T* sxy = (T*)D.sorted_xy;
-        sxy[2 * (size_t)pos] = ((const T*)D.look[0])[i];
-        sxy[2 * (size_t)pos + 1] = ((const T*)D.look[1])[i];
-        D.sorted_hs[pos] = D.hs[i];
+        sxy[2 * (size_t)pos] = gxy[2 * (size_t)k];
+        sxy[2 * (size_t)pos + 1] = gxy[2 * (size_t)k + 1];
+        D.sorted_hs[pos] = D.stage_hs[k];
     }
 }
 

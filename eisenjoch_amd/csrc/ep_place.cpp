@@ -191,9 +191,11 @@ int batch_build(Ctx* c, const std::vector<Design*>& ds, const ep_config* cfg, Ba
 }
 
 void batch_release(Batch& b) {
-    if (b.graph) hipGraphExecDestroy(b.graph);
-    b.graph = nullptr;
-    b.graph_iters = 0;
+    for (int k = 0; k < 2; k++) {
+        if (b.graph[k]) hipGraphExecDestroy(b.graph[k]);
+        b.graph[k] = nullptr;
+        b.graph_iters[k] = 0;
+    }
     if (b.mem) hipFree(b.mem);
     b.mem = nullptr;
     b.ds.clear();
@@ -254,6 +256,7 @@ static const char* kPhaseNames[] = {"sort", "wa", "scatter", "dens_final", "dct_
                                     "dct_i1", "dct_i2", "gather", "ctrl_a", "fix", "ctrl_b",
                                     "update", "hpwl"};
 constexpr int kNumPhases = 14;
+constexpr int kFixPhase = 10;
 
 static void enqueue_phase(Ctx* c, Batch& b, int ph) {
     hipStream_t s = c->stream;
@@ -275,32 +278,50 @@ static void enqueue_phase(Ctx* c, Batch& b, int ph) {
     }
 }
 
-// One iteration of mod.rs:104-197 for every live design of the batch.
-static void enqueue_iteration(Ctx* c, Batch& b) {
-    for (int ph = 0; ph < kNumPhases; ph++) enqueue_phase(c, b, ph);
+// One iteration of mod.rs:104-197 for every live design of the batch.  The
+// fix pass (combine once the penalty was initialised by ctrl_a in the same
+// iteration) can only do work while some design's penalty is unset.
+static void enqueue_iteration(Ctx* c, Batch& b, bool with_fix) {
+    for (int ph = 0; ph < kNumPhases; ph++)
+        if (ph != kFixPhase || with_fix) enqueue_phase(c, b, ph);
+}
+
+// True while a live design of the batch has no penalty yet, from the host
+// mirror of the states (fresh after every reset and every read_states), so
+// a chunk launched without the fix pass never needs it: the penalty is set
+// once and never unset.
+static bool batch_needs_fix(const Ctx* c, const Batch& b) {
+    for (const Design* D : b.ds) {
+        const DevState& s = c->state_host[D->state_idx];
+        if (!s.done && !s.lam_init) return true;
+    }
+    return false;
 }
 
 int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph) {
     if (b.ds.empty() || iters <= 0) return EP_OK;
+    const bool with_fix = batch_needs_fix(c, b);
     if (!use_graph) {
-        for (int i = 0; i < iters; i++) enqueue_iteration(c, b);
+        for (int i = 0; i < iters; i++) enqueue_iteration(c, b, with_fix);
         HIPCHK(c, hipGetLastError());
         return EP_OK;
     }
-    if (!b.graph || b.graph_iters != iters) {
-        if (b.graph) hipGraphExecDestroy(b.graph);
-        b.graph = nullptr;
+    hipGraphExec_t& ge = b.graph[with_fix ? 1 : 0];
+    int& gi = b.graph_iters[with_fix ? 1 : 0];
+    if (!ge || gi != iters) {
+        if (ge) hipGraphExecDestroy(ge);
+        ge = nullptr;
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < iters; i++) enqueue_iteration(c, b);
+        for (int i = 0; i < iters; i++) enqueue_iteration(c, b, with_fix);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(&b.graph, g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
-        b.graph_iters = iters;
+        gi = iters;
     }
-    HIPCHK(c, hipGraphLaunch(b.graph, c->stream));
+    HIPCHK(c, hipGraphLaunch(ge, c->stream));
     return EP_OK;
 }
 

@@ -266,6 +266,8 @@ static size_t plan_arena(Design& D, char* base) {
     G.perm[0] = (uint32_t*)P(N * 4);
     G.perm[1] = (uint32_t*)P(N * 4);
     G.keys = (uint32_t*)P(N * 4);
+    G.stage_xy = P(N * 2 * T);
+    G.stage_hs = (float*)P(N * 4);
     G.sorted_xy = P(N * 2 * T);
     G.sorted_hs = (float*)P(N * 4);
     G.tcount = (uint32_t*)P((size_t)G.n_keys * 4);

@@ -63,8 +63,8 @@ struct Batch {
     DesignDev* d_dev = nullptr;
     const int32_t* dzero = nullptr;  // a device zero: prefix of a one-design view
     BatchDev bd{};
-    hipGraphExec_t graph = nullptr;
-    int graph_iters = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // replayed chunk without / with the fix pass
+    int graph_iters[2] = {0, 0};
     std::vector<double> done_ms;     // per design: host time at which it was seen done
 };
 
