@@ -78,7 +78,7 @@ struct DesignDev {
     void* x0[2];               // initial positions (reset source)
     void* pos[2];              // x, y (solver state)
     void* dpos[2];             // x - x_prev
-    void* look[2];             // clamped look-ahead
+    void* look;                // T [n_mov][2]: clamped look-ahead, x and y interleaved
     void* g[2][2];             // [parity][axis] combined gradient (or density grad)
     void* pg;                  // per-slot WA gradient (gx, gy) in T [n_slots][2]
     uint32_t* cs_off;          // cell -> slot CSR [n_mov + 1]

@@ -91,12 +91,24 @@ __device__ inline T wa_pin(T a, T u, T e, T S, T U, T v, T f, T Q, T V) {
     return e * iS * (T(1) + a * (u - U * iS)) - f * iQ * (T(1) - a * (v - V * iQ));
 }
 
-template <typename T>
-__device__ inline void pin_pos(const DesignDev& D, uint32_t c, const T* lx, const T* ly, T& px,
-                               T& py) {
+// Two-component vector of the arithmetic type (one 8- or 16-byte access).
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { typedef float2 type; };
+template <> struct Vec2<double> { typedef double2 type; };
+
+// Position of pin cell c: the interleaved look-ahead (WA) or the solver
+// positions (HPWL) of a movable cell, else the fixed cell's location.
+template <typename T, bool HPWL>
+__device__ inline void pin_pos(const DesignDev& D, uint32_t c, T& px, T& py) {
     if (c < (uint32_t)D.n_mov) {
-        px = lx[c];
-        py = ly[c];
+        if (HPWL) {
+            px = ((const T*)D.pos[0])[c];
+            py = ((const T*)D.pos[1])[c];
+        } else {
+            const typename Vec2<T>::type v = ((const typename Vec2<T>::type*)D.look)[c];
+            px = v.x;
+            py = v.y;
+        }
     } else {
         px = ((const T*)D.fixed[0])[c - D.n_mov];
         py = ((const T*)D.fixed[1])[c - D.n_mov];
@@ -108,9 +120,11 @@ __device__ inline void pin_pos(const DesignDev& D, uint32_t c, const T* lx, cons
 // deterministic.
 template <typename T>
 __device__ inline void store_grad(const DesignDev& D, uint32_t slot, T gx, T gy) {
-    T* pg = (T*)D.pg;
-    pg[2 * (size_t)slot] = gx;
-    pg[2 * (size_t)slot + 1] = gy;
+    typedef typename Vec2<T>::type V;
+    V v;
+    v.x = gx;
+    v.y = gy;
+    ((V*)D.pg)[slot] = v;
 }
 
 // HPWL mode: per-wave partial of sum over nets of (max - min) x + y of the
@@ -133,10 +147,8 @@ __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
     const int G = D.bucket_group[bk];
     const uint32_t c = D.slots[(size_t)w * kWave + lane];
     const bool valid = c != kPad;
-    const T* lx = (const T*)(HPWL ? D.pos[0] : D.look[0]);
-    const T* ly = (const T*)(HPWL ? D.pos[1] : D.look[1]);
     T px = 0, py = 0;
-    if (valid) pin_pos<T>(D, c, lx, ly, px, py);
+    if (valid) pin_pos<T, HPWL>(D, c, px, py);
     const T big = Lim<T>::big();
     T xmx = valid ? px : -big, xmn = valid ? px : big;
     T ymx = valid ? py : -big, ymn = valid ? py : big;
@@ -185,13 +197,11 @@ __global__ __launch_bounds__(256) void k_wa_hf(BatchDev b, int final_pass) {
     const int e = blockIdx.x - b.pfx_hf[d];
     if (e >= D.n_hf) return;
     const uint32_t p0 = D.hf_off[e], p1 = D.hf_off[e + 1];
-    const T* lx = (const T*)(HPWL ? D.pos[0] : D.look[0]);
-    const T* ly = (const T*)(HPWL ? D.pos[1] : D.look[1]);
     const T big = Lim<T>::big();
     double mm[4] = {-(double)big, -(double)big, -(double)big, -(double)big};  // xmax, -xmin, ymax, -ymin
     for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
         T px, py;
-        pin_pos<T>(D, D.hf_pins[p], lx, ly, px, py);
+        pin_pos<T, HPWL>(D, D.hf_pins[p], px, py);
         mm[0] = fmax(mm[0], (double)px);
         mm[1] = fmax(mm[1], -(double)px);
         mm[2] = fmax(mm[2], (double)py);
@@ -222,7 +232,7 @@ __global__ __launch_bounds__(256) void k_wa_hf(BatchDev b, int final_pass) {
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
         T px, py;
-        pin_pos<T>(D, D.hf_pins[p], lx, ly, px, py);
+        pin_pos<T, HPWL>(D, D.hf_pins[p], px, py);
         T ux = px - xmx, vx = px - xmn, uy = py - ymx, vy = py - ymn;
         T ex = fexp(a * ux), fx = fexp(-a * vx), ey = fexp(a * uy), fy = fexp(-a * vy);
         s[0] += ex; s[1] += ux * ex; s[2] += fx; s[3] += vx * fx;
@@ -242,7 +252,7 @@ __global__ __launch_bounds__(256) void k_wa_hf(BatchDev b, int final_pass) {
         uint32_t c = D.hf_pins[p];
         if (c >= (uint32_t)D.n_mov) continue;
         T px, py;
-        pin_pos<T>(D, c, lx, ly, px, py);
+        pin_pos<T, HPWL>(D, c, px, py);
         T ux = px - xmx, vx = px - xmn, uy = py - ymx, vy = py - ymn;
         T ex = fexp(a * ux), fx = fexp(-a * vx), ey = fexp(a * uy), fy = fexp(-a * vy);
         T gx = wa_pin<T>(a, ux, ex, S[0], S[1], vx, fx, S[2], S[3]);
@@ -303,8 +313,8 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
     if (D.st->done) return;
     const int c0 = (blockIdx.x - b.pfx_scat[d]) * kScatCells;
     const uint32_t* PERM = D.perm[(D.st->it + 1) & 1];
-    const T* LX = (const T*)D.look[0];
-    const T* LY = (const T*)D.look[1];
+    typedef typename Vec2<T>::type V;
+    const V* LXY = (const V*)D.look;
     if (threadIdx.x == 0) {
         box[0] = box[2] = 0x7fffffff;
         box[1] = box[3] = -1;
@@ -314,7 +324,8 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
     for (int k = threadIdx.x; k < kScatCells; k += blockDim.x) {
         if (c0 + k >= D.n_mov) break;
         const int i = PERM[c0 + k];
-        const BinBox r = bin_box(D, (double)LX[i], (double)LY[i], (double)D.hs[i]);
+        const V l = LXY[i];
+        const BinBox r = bin_box(D, (double)l.x, (double)l.y, (double)D.hs[i]);
         if (r.bx1 > r.bx0 && r.by1 > r.by0) {
             lo_x = min(lo_x, r.bx0); hi_x = max(hi_x, r.bx1);
             lo_y = min(lo_y, r.by0); hi_y = max(hi_y, r.by1);
@@ -332,7 +343,8 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
     for (int k = threadIdx.x; k < kScatCells; k += blockDim.x) {
         if (c0 + k >= D.n_mov) break;
         const int i = PERM[c0 + k];
-        const BinBox r = bin_box(D, (double)LX[i], (double)LY[i], (double)D.hs[i]);
+        const V l = LXY[i];
+        const BinBox r = bin_box(D, (double)l.x, (double)l.y, (double)D.hs[i]);
         for (int by = r.by0; by < r.by1; by++)
             for (int bx = r.bx0; bx < r.bx1; bx++) {
                 const double w = ov_area(r, bx, by);
@@ -436,7 +448,8 @@ __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
     uint32_t key = 0u;
     if (valid) {
         const uint32_t i = D.perm[D.st->it & 1][k];
-        const T x = ((const T*)D.look[0])[i], y = ((const T*)D.look[1])[i];
+        const typename Vec2<T>::type l = ((const typename Vec2<T>::type*)D.look)[i];
+        const T x = l.x, y = l.y;
         const float hs = D.hs[i];
         key = tile_key<T>(D, x, y, hs);
         D.keys[k] = key;
@@ -581,15 +594,35 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
 
 // ------------------------------------------------------------ gather ----
 
-// WA gradient of cell i: its pin slots summed in ascending slot order.
+// WA gradient of cell i: its pin slots summed in ascending slot order.  The
+// slot indices and slot values of four pins are loaded before any is added,
+// so four dependent load pairs are in flight at once; the additions keep
+// the sequential order.
 template <typename T>
 __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy) {
-    const T* pg = (const T*)D.pg;
+    typedef typename Vec2<T>::type V;
+    const V* pg = (const V*)D.pg;
     double sx = 0.0, sy = 0.0;
-    for (uint32_t k = D.cs_off[i], e = D.cs_off[i + 1]; k < e; k++) {
-        const size_t s = D.cs_idx[k];
-        sx += (double)pg[2 * s];
-        sy += (double)pg[2 * s + 1];
+    uint32_t k = D.cs_off[i];
+    const uint32_t e = D.cs_off[i + 1];
+    for (; k + 4 <= e; k += 4) {
+        const uint32_t s0 = D.cs_idx[k], s1 = D.cs_idx[k + 1];
+        const uint32_t s2 = D.cs_idx[k + 2], s3 = D.cs_idx[k + 3];
+        const V v0 = pg[s0], v1 = pg[s1], v2 = pg[s2], v3 = pg[s3];
+        sx += (double)v0.x; sy += (double)v0.y;
+        sx += (double)v1.x; sy += (double)v1.y;
+        sx += (double)v2.x; sy += (double)v2.y;
+        sx += (double)v3.x; sy += (double)v3.y;
+    }
+    const int rest = (int)(e - k);
+    if (rest > 0) {
+        const uint32_t s0 = D.cs_idx[k];
+        const uint32_t s1 = rest > 1 ? D.cs_idx[k + 1] : s0;
+        const uint32_t s2 = rest > 2 ? D.cs_idx[k + 2] : s0;
+        const V v0 = pg[s0], v1 = pg[s1], v2 = pg[s2];
+        sx += (double)v0.x; sy += (double)v0.y;
+        if (rest > 1) { sx += (double)v1.x; sy += (double)v1.y; }
+        if (rest > 2) { sx += (double)v2.x; sy += (double)v2.y; }
     }
     gx = sx;
     gy = sy;
@@ -612,22 +645,49 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
     if (i < D.n_mov) {
         double gden[2] = {0, 0};
         if (MODE != 1) {
-            const double cx = (double)((const T*)D.look[0])[i];
-            const double cy = (double)((const T*)D.look[1])[i];
+            const typename Vec2<T>::type l = ((const typename Vec2<T>::type*)D.look)[i];
+            const double cx = (double)l.x, cy = (double)l.y;
             const BinBox r = bin_box(D, cx, cy, (double)D.hs[i]);
             const T* E0 = (const T*)D.field[0];
             const T* E1 = (const T*)D.field[1];
             double fx = 0, fy = 0, wt = 0;
-            for (int by = r.by0; by < r.by1; by++)
-                for (int bx = r.bx0; bx < r.bx1; bx++) {
-                    double w = ov_area(r, bx, by);
-                    if (w > 0.0) {
-                        size_t idx = (size_t)by * D.Bx + bx;
-                        fx += w * (double)E0[idx];
-                        fy += w * (double)E1[idx];
-                        wt += w;
+            if (r.bx1 - r.bx0 <= 3 && r.by1 - r.by0 <= 3) {
+                // footprints of up to 3x3 bins (all of them at areas <= 4 and
+                // bins no smaller than tiles): the nine field pairs are loaded
+                // together, then added in the same row-major order as below
+                T e0[9], e1[9];
+#pragma unroll
+                for (int q = 0; q < 9; q++) {
+                    const int by = r.by0 + q / 3, bx = r.bx0 + q % 3;
+                    const bool in = by < r.by1 && bx < r.bx1;
+                    const size_t idx = in ? (size_t)by * D.Bx + bx : 0;
+                    e0[q] = in ? E0[idx] : T(0);
+                    e1[q] = in ? E1[idx] : T(0);
+                }
+#pragma unroll
+                for (int q = 0; q < 9; q++) {
+                    const int by = r.by0 + q / 3, bx = r.bx0 + q % 3;
+                    if (by < r.by1 && bx < r.bx1) {
+                        double w = ov_area(r, bx, by);
+                        if (w > 0.0) {
+                            fx += w * (double)e0[q];
+                            fy += w * (double)e1[q];
+                            wt += w;
+                        }
                     }
                 }
+            } else {
+                for (int by = r.by0; by < r.by1; by++)
+                    for (int bx = r.bx0; bx < r.bx1; bx++) {
+                        double w = ov_area(r, bx, by);
+                        if (w > 0.0) {
+                            size_t idx = (size_t)by * D.Bx + bx;
+                            fx += w * (double)E0[idx];
+                            fy += w * (double)E1[idx];
+                            wt += w;
+                        }
+                    }
+            }
             if (wt > 1e-10) {
                 gden[0] = fx / wt;
                 gden[1] = fy / wt;
@@ -832,6 +892,7 @@ __global__ __launch_bounds__(256) void k_update(BatchDev b) {
     double sq[2] = {0, 0};
     if (i < D.n_mov) {
         const T m = (T)st->m, mn = (T)st->m_next;
+        T lk[2];
 #pragma unroll
         for (int ax = 0; ax < 2; ax++) {
             const T hi = (T)(ax == 0 ? D.max_x : D.max_y);
@@ -851,8 +912,12 @@ __global__ __launch_bounds__(256) void k_update(BatchDev b) {
             DP[i] = ndx;
             T l = xnc + mn * ndx;  // nesterov.rs:95-101 then common.rs:410-415
             l = l < T(0) ? T(0) : (l > hi ? hi : l);
-            ((T*)D.look[ax])[i] = l;
+            lk[ax] = l;
         }
+        typename Vec2<T>::type v;
+        v.x = lk[0];
+        v.y = lk[1];
+        ((typename Vec2<T>::type*)D.look)[i] = v;
     }
     block_sum<2>(sq, sh);
     if (threadIdx.x == 0) {
@@ -870,14 +935,19 @@ __global__ __launch_bounds__(256) void k_reset(BatchDev b) {
     const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
     D.perm[0][i] = (uint32_t)i;
+    T lk[2];
 #pragma unroll
     for (int ax = 0; ax < 2; ax++) {
         const T hi = (T)(ax == 0 ? D.max_x : D.max_y);
         const T x = ((const T*)D.x0[ax])[i];
         ((T*)D.pos[ax])[i] = x;
         ((T*)D.dpos[ax])[i] = T(0);
-        ((T*)D.look[ax])[i] = x < T(0) ? T(0) : (x > hi ? hi : x);
+        lk[ax] = x < T(0) ? T(0) : (x > hi ? hi : x);
     }
+    typename Vec2<T>::type v;
+    v.x = lk[0];
+    v.y = lk[1];
+    ((typename Vec2<T>::type*)D.look)[i] = v;
 }
 
 // Evaluation helper: the WA gradient per cell (the gather's slot reduction).
@@ -901,11 +971,11 @@ __global__ __launch_bounds__(256) void k_bin_ranges(BatchDev b, int32_t* out) {
     if (i >= D.n_mov) return;
     double cx, cy;
     if (D.prec == EP_PREC_FP64) {
-        cx = ((const double*)D.look[0])[i];
-        cy = ((const double*)D.look[1])[i];
+        cx = ((const double*)D.look)[2 * (size_t)i];
+        cy = ((const double*)D.look)[2 * (size_t)i + 1];
     } else {
-        cx = (double)((const float*)D.look[0])[i];
-        cy = (double)((const float*)D.look[1])[i];
+        cx = (double)((const float*)D.look)[2 * (size_t)i];
+        cy = (double)((const float*)D.look)[2 * (size_t)i + 1];
     }
     BinBox r = bin_box(D, cx, cy, (double)D.hs[i]);
     out[4 * i + 0] = r.bx0;

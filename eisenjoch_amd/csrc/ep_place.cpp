@@ -733,6 +733,16 @@ static int put_T(Ctx* c, void* dst, const double* src, size_t n, int prec) {
     return EP_OK;
 }
 
+// x and y interleaved into a T [n][2] device array.
+static int put_xy(Ctx* c, void* dst, const double* x, const double* y, size_t n, int prec) {
+    std::vector<double> xy(2 * n);
+    for (size_t i = 0; i < n; i++) {
+        xy[2 * i] = x[i];
+        xy[2 * i + 1] = y[i];
+    }
+    return put_T(c, dst, xy.data(), 2 * n, prec);
+}
+
 static int get_T(Ctx* c, double* dst, const void* src, size_t n, int prec) {
     if (!n) return EP_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -770,8 +780,7 @@ int ep_eval_wa_grad(ep_ctx* h, uint32_t id, const double* x, const double* y, do
     int rc = eval_prep(c, id, &D, &b);
     if (rc) return rc;
     const size_t N = (size_t)D->dev.n_mov;
-    if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
-    if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
+    if ((rc = put_xy(c, D->dev.look, x, y, N, D->prec))) return rc;
     if (!N) return EP_OK;
     double* dbuf = nullptr;
     HIPCHK(c, hipMalloc(&dbuf, N * 16));
@@ -795,8 +804,7 @@ int ep_eval_density(ep_ctx* h, uint32_t id, const double* x, const double* y, do
     int rc = eval_prep(c, id, &D, &b);
     if (rc) return rc;
     const size_t N = (size_t)D->dev.n_mov, NB = (size_t)D->dev.Bx * D->dev.By;
-    if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
-    if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
+    if ((rc = put_xy(c, D->dev.look, x, y, N, D->prec))) return rc;
     launch_tile_sort(b->bd, b->prec, b->cell_blocks, c->stream);
     launch_scatter_pull(b->bd, b->prec, b->tile_blocks, c->stream);
     launch_scatter(b->bd, b->prec, b->scat_blocks, c->stream);
@@ -826,8 +834,7 @@ int ep_eval_bin_ranges(ep_ctx* h, uint32_t id, const double* x, const double* y,
     if (rc) return rc;
     const size_t N = (size_t)D->dev.n_mov;
     if (!N) return EP_OK;
-    if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
-    if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
+    if ((rc = put_xy(c, D->dev.look, x, y, N, D->prec))) return rc;
     int32_t* dbuf = nullptr;
     HIPCHK(c, hipMalloc(&dbuf, N * 16));
     launch_bin_ranges(b->bd, b->cell_blocks, dbuf, c->stream);
@@ -868,8 +875,7 @@ int ep_eval_density_grad(ep_ctx* h, uint32_t id, const double* x, const double* 
     if (rc) return rc;
     const size_t N = (size_t)D->dev.n_mov, NB = (size_t)D->dev.Bx * D->dev.By;
     if (!N) return EP_OK;
-    if ((rc = put_T(c, D->dev.look[0], x, N, D->prec))) return rc;
-    if ((rc = put_T(c, D->dev.look[1], y, N, D->prec))) return rc;
+    if ((rc = put_xy(c, D->dev.look, x, y, N, D->prec))) return rc;
     if ((rc = put_T(c, D->dev.field[0], ex, NB, D->prec))) return rc;
     if ((rc = put_T(c, D->dev.field[1], ey, NB, D->prec))) return rc;
     double* dbuf = nullptr;

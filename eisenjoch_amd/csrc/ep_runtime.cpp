@@ -257,7 +257,7 @@ static size_t plan_arena(Design& D, char* base) {
     for (int ax = 0; ax < 2; ax++) G.x0[ax] = P(N * T);
     for (int ax = 0; ax < 2; ax++) G.pos[ax] = P(N * T);
     for (int ax = 0; ax < 2; ax++) G.dpos[ax] = P(N * T);
-    for (int ax = 0; ax < 2; ax++) G.look[ax] = P(N * T);
+    G.look = P(N * 2 * T);
     for (int p = 0; p < 2; p++)
         for (int ax = 0; ax < 2; ax++) G.g[p][ax] = P(N * T);
     G.pg = P((size_t)G.n_slots * 2 * T);
