@@ -114,9 +114,6 @@ __global__ __launch_bounds__(256) void k_gemm_fma(const GemmJob* jobs, int nj, c
 
 // -------------------------------------------------------------- MFMA -----
 
-constexpr int kTK = 32;        // K per LDS stage
-constexpr int kLdk = kTK + 8;  // padded row (bf16 elements): 80 bytes
-
 __device__ inline void split8(const float* v, bf16x8& hi, bf16x8& lo) {
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -126,48 +123,63 @@ __device__ inline void split8(const float* v, bf16x8& hi, bf16x8& lo) {
     }
 }
 
-// One thread's share of a 64 x 32 operand tile: 8 consecutive K values of
-// one row, held as BF16 hi/lo in registers between the global load and the
-// LDS store (so the next stage's loads overlap this stage's MFMAs).
+// One thread's share of a 64 x TK operand tile: TK / 32 chunks of 8
+// consecutive K values of one row (chunk c at K offset 8 (tid % 4) + 32 c),
+// held as BF16 hi/lo in registers between the global load and the LDS store
+// (so the next stage's loads overlap this stage's MFMAs).
+template <int TK>
 struct Frag {
-    bf16x8 hi, lo;
+    bf16x8 hi[TK / 32], lo[TK / 32];
 };
 
-__device__ inline Frag load_frag(int kind, const void* p, const void* plo, int r0, int k0, int K) {
+template <int TK>
+__device__ inline Frag<TK> load_frag(int kind, const void* p, const void* plo, int r0, int k0, int K) {
     const int tid = threadIdx.x;
     const int row = tid >> 2, kc = (tid & 3) * 8;
-    const size_t off = (size_t)(r0 + row) * K + k0 + kc;
-    Frag f;
-    if (kind == 0) {
-        const float4* src = (const float4*)((const float*)p + off);
-        const float4 a = src[0], b = src[1];
-        const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        split8(v, f.hi, f.lo);
-    } else {
-        f.hi = *(const bf16x8*)((const __bf16*)p + off);
-        f.lo = *(const bf16x8*)((const __bf16*)plo + off);
+    Frag<TK> f;
+#pragma unroll
+    for (int c = 0; c < TK / 32; c++) {
+        const size_t off = (size_t)(r0 + row) * K + k0 + kc + 32 * c;
+        if (kind == 0) {
+            const float4* src = (const float4*)((const float*)p + off);
+            const float4 a = src[0], b = src[1];
+            const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            split8(v, f.hi[c], f.lo[c]);
+        } else {
+            f.hi[c] = *(const bf16x8*)((const __bf16*)p + off);
+            f.lo[c] = *(const bf16x8*)((const __bf16*)plo + off);
+        }
     }
     return f;
 }
 
-__device__ inline void store_frag(__bf16 (*s)[64][kLdk], const Frag& f) {
+template <int TK>
+__device__ inline void store_frag(__bf16 (*s)[64][TK + 8], const Frag<TK>& f) {
     const int tid = threadIdx.x;
     const int row = tid >> 2, kc = (tid & 3) * 8;
-    *(bf16x8*)&s[0][row][kc] = f.hi;
-    *(bf16x8*)&s[1][row][kc] = f.lo;
+#pragma unroll
+    for (int c = 0; c < TK / 32; c++) {
+        *(bf16x8*)&s[0][row][kc + 32 * c] = f.hi[c];
+        *(bf16x8*)&s[1][row][kc + 32 * c] = f.lo[c];
+    }
 }
 
 // 64 x 64 output tile per block, four waves each owning a 32 x 32 quadrant,
-// K in stages of 32 through two LDS buffers.  Requires M, N multiples of 64
-// and K a multiple of 32 (checked on the host).
+// K in stages of TK through two LDS buffers (rows padded by 8 BF16), with
+// the global loads two stages ahead of the MFMAs: the chain of stages bounds
+// a large transform, and a stage otherwise waits a whole load latency.  TK =
+// 64 halves the chain at twice the LDS (half the resident blocks).
+// Requires M, N multiples of 64 and K a multiple of TK (checked on the host).
+template <int TK>
 __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, const DesignDev* dd) {
-    __shared__ __bf16 sA[2][2][64][kLdk];  // [buffer][hi/lo][row][k]
-    __shared__ __bf16 sB[2][2][64][kLdk];
-    const int j = find_job(jobs, nj, blockIdx.x);
+    __shared__ __bf16 sA[2][2][64][TK + 8];  // [buffer][hi/lo][row][k]
+    __shared__ __bf16 sB[2][2][64][TK + 8];
+    const int tb = blockIdx.x;
+    const int j = find_job(jobs, nj, tb);
     const GemmJob& J = jobs[j];
     const DesignDev& D = dd[J.design];
     if (D.st->done) return;
-    const int t = blockIdx.x - J.tile_start;
+    const int t = tb - J.tile_start;
     const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int wm = wv >> 1, wn = wv & 1;
@@ -177,21 +189,29 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, 
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = 0.0f;
     const int r = lane & 31, kh = (lane >> 5) * 8;
-    Frag fa = load_frag(J.a_kind, pa, J.A_lo, m0, 0, J.K);
-    Frag fb = load_frag(J.b_kind, pb, J.B_lo, n0, 0, J.K);
-    store_frag(sA[0], fa);
-    store_frag(sB[0], fb);
+    const int stages = J.K / TK;
+    // stage 0 goes to LDS; stage 1 waits in registers (fa, fb); each stage
+    // then issues the loads of the stage two ahead (na, nb), so two stages'
+    // loads are in flight while one is multiplied
+    Frag<TK> fa = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, 0, J.K);
+    Frag<TK> fb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, 0, J.K);
+    store_frag<TK>(sA[0], fa);
+    store_frag<TK>(sB[0], fb);
+    if (stages > 1) {
+        fa = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, TK, J.K);
+        fb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, TK, J.K);
+    }
     __syncthreads();
-    const int stages = J.K / kTK;
     for (int st = 0; st < stages; st++) {
         const int cur = st & 1;
         const bool more = st + 1 < stages;
-        if (more) {
-            fa = load_frag(J.a_kind, pa, J.A_lo, m0, (st + 1) * kTK, J.K);
-            fb = load_frag(J.b_kind, pb, J.B_lo, n0, (st + 1) * kTK, J.K);
+        Frag<TK> na, nb;
+        if (st + 2 < stages) {
+            na = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, (st + 2) * TK, J.K);
+            nb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, (st + 2) * TK, J.K);
         }
 #pragma unroll
-        for (int kk = 0; kk < kTK; kk += 16) {
+        for (int kk = 0; kk < TK; kk += 16) {
             const bf16x8 ah = *(const bf16x8*)&sA[cur][0][wm * 32 + r][kk + kh];
             const bf16x8 al = *(const bf16x8*)&sA[cur][1][wm * 32 + r][kk + kh];
             const bf16x8 bh = *(const bf16x8*)&sB[cur][0][wn * 32 + r][kk + kh];
@@ -203,10 +223,12 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, 
         if (more) {
             // the other buffer was last read in stage st - 1, before the
             // barrier that ended it, so it is free to overwrite
-            store_frag(sA[cur ^ 1], fa);
-            store_frag(sB[cur ^ 1], fb);
+            store_frag<TK>(sA[cur ^ 1], fa);
+            store_frag<TK>(sB[cur ^ 1], fb);
         }
         __syncthreads();
+        fa = na;
+        fb = nb;
     }
     // accumulator layout: element q of lane l is (row 8*(q/4) + 4*(l/32) + q%4, col l%32)
 #pragma unroll
@@ -298,7 +320,9 @@ void launch_gemm_pass(const GemmJob* jobs, int n_jobs, int total_tiles, const De
     if (prec == EP_PREC_FP64)
         hipLaunchKernelGGL(k_gemm_fma<double>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
     else if (engine == 0)
-        hipLaunchKernelGGL(k_gemm_mfma, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+        hipLaunchKernelGGL(k_gemm_mfma<32>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+    else if (engine == 3)
+        hipLaunchKernelGGL(k_gemm_mfma<64>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
     else if (engine == 2)
         hipLaunchKernelGGL(k_gemm_mfma_sk, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
     else

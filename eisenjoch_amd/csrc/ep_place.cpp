@@ -53,8 +53,8 @@ int validate_config(Ctx* c, const ep_config* cfg, int log_cap) {
 }
 
 // Transform engine of a design shape: the autotuned choice when there is
-// one, else MFMA with LDS staging for BF16-eligible shapes (bins multiples of
-// 64 on the FAST path) and the FMA kernel otherwise.
+// one, else MFMA with LDS staging in K stages of 32 for BF16-eligible shapes
+// (bins multiples of 64 on the FAST path) and the FMA kernel otherwise.
 int engine_for(Ctx* c, int prec, int Bx, int By) {
     const bool mfma_ok = prec == EP_PREC_FAST && Bx % 64 == 0 && By % 64 == 0 &&
                          getenv("EP_FORCE_FMA") == nullptr;
@@ -256,6 +256,8 @@ static const char* kPhaseNames[] = {"sort", "wa", "scatter", "dens_final", "dct_
                                     "dct_i1", "dct_i2", "gather", "ctrl_a", "fix", "ctrl_b",
                                     "update", "hpwl"};
 constexpr int kNumPhases = 14;
+constexpr int kWaPhase = 1;
+constexpr int kGatherPhase = 8;
 constexpr int kFixPhase = 10;
 
 static void enqueue_phase(Ctx* c, Batch& b, int ph) {
@@ -279,11 +281,24 @@ static void enqueue_phase(Ctx* c, Batch& b, int ph) {
 }
 
 // One iteration of mod.rs:104-197 for every live design of the batch.  The
-// fix pass (combine once the penalty was initialised by ctrl_a in the same
-// iteration) can only do work while some design's penalty is unset.
+// WA gradient depends only on the look-ahead positions, as the density chain
+// (sort, scatter, transforms) does, and the two meet at the gather: the WA
+// kernels run on the side stream, forked after the previous update and
+// joined before the gather, so they fill the CUs the latency-bound density
+// chain leaves idle (in a captured chunk the fork and join become graph
+// edges).  The fix pass (combine once the penalty was initialised by ctrl_a
+// in the same iteration) can only do work while some design's penalty is
+// unset.
 static void enqueue_iteration(Ctx* c, Batch& b, bool with_fix) {
-    for (int ph = 0; ph < kNumPhases; ph++)
-        if (ph != kFixPhase || with_fix) enqueue_phase(c, b, ph);
+    hipEventRecord(c->ev_fork, c->stream);
+    hipStreamWaitEvent(c->side, c->ev_fork, 0);
+    launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, c->side);
+    hipEventRecord(c->ev_join, c->side);
+    for (int ph = 0; ph < kNumPhases; ph++) {
+        if (ph == kWaPhase || (ph == kFixPhase && !with_fix)) continue;
+        if (ph == kGatherPhase) hipStreamWaitEvent(c->stream, c->ev_join, 0);
+        enqueue_phase(c, b, ph);
+    }
 }
 
 // True while a live design of the batch has no penalty yet, from the host
@@ -563,7 +578,7 @@ extern "C" int ep_autotune(ep_ctx* h, const uint32_t* ids, uint32_t n, uint32_t 
         }
         double best = 1e300;
         int best_e = 0;
-        for (int e : {0, 2, 1}) {
+        for (int e : {3, 0, 2, 1}) {
             c->tune[kv.first] = e;
             release_batch_cache(c);
             Batch b;

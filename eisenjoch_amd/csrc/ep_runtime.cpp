@@ -411,6 +411,9 @@ int ep_create(ep_ctx** out, int device, uint64_t budget) {
     HIPCHK(nullptr, hipMemGetInfo(&fr, &tot));
     c->budget = budget ? budget : (uint64_t)(tot * 0.9);
     HIPCHK(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(nullptr, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIPCHK(nullptr, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHK(nullptr, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     HIPCHK(nullptr, hipMalloc(&c->state_pool, sizeof(DevState) * kStatePool));
     HIPCHK(nullptr, hipHostMalloc(&c->state_host, sizeof(DevState) * kStatePool, hipHostMallocDefault));
     for (int i = kStatePool - 1; i >= 0; i--) c->free_states.push_back(i);
@@ -435,6 +438,10 @@ void ep_destroy(ep_ctx* h) {
     release_batch_cache(c);
     hipFree(c->state_pool);
     hipHostFree(c->state_host);
+    hipStreamSynchronize(c->side);
+    hipEventDestroy(c->ev_fork);
+    hipEventDestroy(c->ev_join);
+    hipStreamDestroy(c->side);
     hipStreamDestroy(c->stream);
     delete c;
 }

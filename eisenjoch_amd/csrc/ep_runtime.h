@@ -18,8 +18,9 @@ struct ep_ctx {};
 namespace ep {
 
 constexpr int kStatePool = 8192;      // resident designs per context
-// Transform engines: 0 MFMA with LDS staging, 1 FP32/FP64 FMA, 2 MFMA split-K.
-constexpr int kEngines = 3;
+// Transform engines: 0 MFMA with LDS staging in K stages of 32, 1 FP32/FP64
+// FMA, 2 MFMA split-K, 3 MFMA with LDS staging in K stages of 64.
+constexpr int kEngines = 4;
 constexpr int kDefaultLogCap = 2048;  // iteration records per design
 
 // Constant transform matrices of one size and field mode: [0] C2, [1] C3,
@@ -79,6 +80,8 @@ struct Ctx : ep_ctx {
     uint64_t used = 0;
     uint64_t transform_bytes = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;      // WA gradient, concurrent with the density chain
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::recursive_mutex mu;     // serialises device work
     std::mutex err_mu;
     std::string last_error, err_copy;

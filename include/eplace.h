@@ -198,7 +198,8 @@ int ep_profile_iteration(ep_ctx* ctx, const uint32_t* ids, uint32_t n, const ep_
                          uint32_t* n_out);
 
 /* Autotuner (no counterpart in the reference): times every eligible
- * transform engine (MFMA with LDS staging, MFMA split-K, FP32 FMA) on one
+ * transform engine (engine 0: MFMA with LDS staging in K stages of 32;
+ * 1: FP32 FMA; 2: MFMA split-K; 3: MFMA with K stages of 64) on one
  * resident FAST design of each distinct bin shape among `ids`, `reps` times,
  * and keeps the fastest per shape for every later batch.  With a cache path,
  * shapes found in the file are reused and new results appended. */
