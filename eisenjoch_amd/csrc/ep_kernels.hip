@@ -463,34 +463,35 @@ __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
 
 // One block per design: tstart = exclusive scan of the counts (with the
 // total at n_keys), cursors = tstart, counts cleared for the next sort.
+// Each thread owns a contiguous run of keys; the run totals are scanned
+// with wave shuffles and one LDS step across the four waves.
 __global__ __launch_bounds__(256) void k_tile_scan(BatchDev b) {
-    __shared__ uint32_t part[256];
+    __shared__ uint32_t wsum[4];
     const DesignDev& D = b.d[blockIdx.x];
     if (D.st->done) return;
     const int n = D.n_keys;
     const int per = (n + 255) / 256;
-    const int lo = min(threadIdx.x * per, n), hi = min(lo + per, n);
+    const int lo = min((int)threadIdx.x * per, n), hi = min(lo + per, n);
     uint32_t sum = 0;
     for (int k = lo; k < hi; k++) sum += D.tcount[k];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int t = 0; t < 256; t++) {
-            const uint32_t v = part[t];
-            part[t] = run;
-            run += v;
-        }
-        D.tstart[n] = run;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t v = sum;  // inclusive scan of the run totals within the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, kWave);
+        if (lane >= o) v += t;
     }
+    if (lane == 63) wsum[w] = v;
     __syncthreads();
-    uint32_t run = part[threadIdx.x];
+    uint32_t base = 0;
+    for (int q = 0; q < w; q++) base += wsum[q];
+    if (threadIdx.x == 255) D.tstart[n] = base + v;
+    uint32_t run = base + v - sum;
     for (int k = lo; k < hi; k++) {
-        const uint32_t v = D.tcount[k];
+        const uint32_t c = D.tcount[k];
         D.tstart[k] = run;
         D.tcur[k] = run;
         D.tcount[k] = 0u;
-        run += v;
+        run += c;
     }
 }
 
@@ -749,20 +750,34 @@ __device__ inline void bsum_range(const double* p, int n, int stride, int off, d
     __syncthreads();
 }
 
+// NV columns of a strided partials array in one pass, each summed in the
+// same order as bsum_range (so bit for bit the same); the sums are valid in
+// thread 0 only.
+template <int NV>
+__device__ inline void bsum_cols(const double* p, int n, int stride, int off0, double (&out)[NV],
+                                 double* sh) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) out[q] = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NV; q++) out[q] += p[(size_t)k * stride + off0 + q];
+    block_sum<NV>(out, sh);
+    __syncthreads();
+}
+
 // Reduce the partials of the previous update (step norms, HPWL) into its log record.
 __device__ inline void reduce_prev(const DesignDev& D, DevState* st, double* sh) {
     if (!st->prev_pending) return;
-    double sx, sy;
-    bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 4, sx, sh);
-    bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 5, sy, sh);
+    double sq[2];
+    bsum_cols<2>(D.part_cell, D.n_cell_blocks, kCellParts, 4, sq, sh);
     double h = 0.0;
     bool hp = st->prev_check && D.cfg.log_hpwl;
     if (hp) bsum_range(D.part_net, D.n_wa_waves + D.n_hf, 1, 0, h, sh);
     if (threadIdx.x == 0) {
         const int k = st->it - 1;
         if (k >= 0 && k < D.cfg.max_iters) {
-            D.log[k].step_norm_x = sqrt(sx);
-            D.log[k].step_norm_y = sqrt(sy);
+            D.log[k].step_norm_x = sqrt(sq[0]);
+            D.log[k].step_norm_y = sqrt(sq[1]);
             D.log[k].hpwl = hp ? h : 0.0;
         }
         if (hp) st->hpwl = h;
@@ -771,82 +786,63 @@ __device__ inline void reduce_prev(const DesignDev& D, DevState* st, double* sh)
     __syncthreads();
 }
 
-// which = 0: after the gather; 1: after the fix (end of control);
-// 2: finalize pending log partials; 3: reduce a final HPWL pass.
-__global__ __launch_bounds__(256) void k_ctrl(BatchDev b, int which) {
-    __shared__ double sh[32];
-    const DesignDev& D = b.d[blockIdx.x];
-    DevState* st = D.st;
+// After the gather (mod.rs:131-143, 159-166): overflow, penalty schedule or
+// initialisation, BB step sizes, the iteration's log record.
+__device__ inline void ctrl_after_gather(const DesignDev& D, DevState* st, double* sh) {
     const DevCfg& c = D.cfg;
-    if (which == 2) {
-        reduce_prev(D, st, sh);
-        return;
-    }
-    if (which == 3) {
-        double h;
-        bsum_range(D.part_net, D.n_wa_waves + D.n_hf, 1, 0, h, sh);
-        if (threadIdx.x == 0) st->final_hpwl = h;
-        return;
-    }
-    if (st->done) {
-        if (which == 1 && threadIdx.x == 0) st->run_update = 0;
-        return;
-    }
     const int k = st->it;
-    if (which == 0) {
-        reduce_prev(D, st, sh);
-        double tot, ovs, p0, p1, p2, p3;
-        bsum_range(D.part_bin, D.n_ov_parts, kBinParts, 0, tot, sh);
-        bsum_range(D.part_bin, D.n_ov_parts, kBinParts, 1, ovs, sh);
-        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 0, p0, sh);
-        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 1, p1, sh);
-        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 2, p2, sh);
-        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 3, p3, sh);
-        if (threadIdx.x != 0) return;
-        const double ov = tot < 1e-30 ? 0.0 : ovs / tot;  // density.rs:79-86
-        st->ov = ov;
-        double wln = 0.0, dnn = 0.0;
-        if (st->lam_init) {
-            st->lam = sched(c, st->lam);
-            if (k > 0) {  // mod.rs:159-166 with nesterov.rs:199-218
-                if (fabs(p1) >= 1e-30) st->s[0] = fmin(fmax(fabs(p0 / p1), 1e-4), 1.0);
-                if (fabs(p3) >= 1e-30) st->s[1] = fmin(fmax(fabs(p2 / p3), 1e-4), 1.0);
-            }
-            st->need_fix = 0;
-        } else {
-            wln = sqrt(p0);  // common.rs:418-425
-            dnn = sqrt(p1);
-            if (dnn > 1e-30) {  // mod.rs:132-138
-                st->lam = c.eta * wln / dnn;
-                st->lam_init = 1;
-            }
-            st->need_fix = 1;
-        }
-        st->wl_norm = wln;
-        st->den_norm = dnn;
-        st->check = c.stop_every_iter || (k % c.interval == 0) || (k == c.max_iters - 1);
-        if (c.check_nonfinite && !(isfinite(ov) && isfinite(st->lam))) st->nonfinite = 1;
-        ep_iter_rec& r = D.log[k];
-        r.iter = k;
-        r.checked = st->check;
-        r.overflow = ov;
-        r.lambda = st->lam;
-        r.wl_norm = wln;
-        r.den_norm = dnn;
-        r.step_norm_x = r.step_norm_y = 0.0;
-        r.hpwl = 0.0;
-        return;
-    }
-    // which == 1
-    if (st->need_fix) {
-        double p0, p1, p2, p3;
-        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 0, p0, sh);
-        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 1, p1, sh);
-        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 2, p2, sh);
-        bsum_range(D.part_cell, D.n_cell_blocks, kCellParts, 3, p3, sh);
-        if (threadIdx.x == 0 && k > 0) {
+    reduce_prev(D, st, sh);
+    double bins[2], cells[4];
+    bsum_cols<2>(D.part_bin, D.n_ov_parts, kBinParts, 0, bins, sh);
+    bsum_cols<4>(D.part_cell, D.n_cell_blocks, kCellParts, 0, cells, sh);
+    if (threadIdx.x != 0) return;
+    const double tot = bins[0], ovs = bins[1];
+    const double p0 = cells[0], p1 = cells[1], p2 = cells[2], p3 = cells[3];
+    const double ov = tot < 1e-30 ? 0.0 : ovs / tot;  // density.rs:79-86
+    st->ov = ov;
+    double wln = 0.0, dnn = 0.0;
+    if (st->lam_init) {
+        st->lam = sched(c, st->lam);
+        if (k > 0) {  // mod.rs:159-166 with nesterov.rs:199-218
             if (fabs(p1) >= 1e-30) st->s[0] = fmin(fmax(fabs(p0 / p1), 1e-4), 1.0);
             if (fabs(p3) >= 1e-30) st->s[1] = fmin(fmax(fabs(p2 / p3), 1e-4), 1.0);
+        }
+        st->need_fix = 0;
+    } else {
+        wln = sqrt(p0);  // common.rs:418-425
+        dnn = sqrt(p1);
+        if (dnn > 1e-30) {  // mod.rs:132-138
+            st->lam = c.eta * wln / dnn;
+            st->lam_init = 1;
+        }
+        st->need_fix = 1;
+    }
+    st->wl_norm = wln;
+    st->den_norm = dnn;
+    st->check = c.stop_every_iter || (k % c.interval == 0) || (k == c.max_iters - 1);
+    if (c.check_nonfinite && !(isfinite(ov) && isfinite(st->lam))) st->nonfinite = 1;
+    ep_iter_rec& r = D.log[k];
+    r.iter = k;
+    r.checked = st->check;
+    r.overflow = ov;
+    r.lambda = st->lam;
+    r.wl_norm = wln;
+    r.den_norm = dnn;
+    r.step_norm_x = r.step_norm_y = 0.0;
+    r.hpwl = 0.0;
+}
+
+// End of the control step (nesterov.rs:41-47, mod.rs:176-195): BB steps of
+// an iteration whose penalty was just initialised, FISTA sequence, stop rule.
+__device__ inline void ctrl_end(const DesignDev& D, DevState* st, double* sh) {
+    const DevCfg& c = D.cfg;
+    const int k = st->it;
+    if (st->need_fix) {
+        double cells[4];
+        bsum_cols<4>(D.part_cell, D.n_cell_blocks, kCellParts, 0, cells, sh);
+        if (threadIdx.x == 0 && k > 0) {
+            if (fabs(cells[1]) >= 1e-30) st->s[0] = fmin(fmax(fabs(cells[0] / cells[1]), 1e-4), 1.0);
+            if (fabs(cells[3]) >= 1e-30) st->s[1] = fmin(fmax(fabs(cells[2] / cells[3]), 1e-4), 1.0);
         }
     }
     if (threadIdx.x != 0) return;
@@ -875,6 +871,32 @@ __global__ __launch_bounds__(256) void k_ctrl(BatchDev b, int which) {
     st->it = k + 1;
     st->n_log = k + 1;
     if (stop) st->done = 1;
+}
+
+// which = 0: after the gather; 1: after the fix (end of control); 4: both
+// in one launch (iterations without a fix pass); 2: finalize pending log
+// partials; 3: reduce a final HPWL pass.  One block per design.
+__global__ __launch_bounds__(256) void k_ctrl(BatchDev b, int which) {
+    __shared__ double sh[32];
+    const DesignDev& D = b.d[blockIdx.x];
+    DevState* st = D.st;
+    if (which == 2) {
+        reduce_prev(D, st, sh);
+        return;
+    }
+    if (which == 3) {
+        double h;
+        bsum_range(D.part_net, D.n_wa_waves + D.n_hf, 1, 0, h, sh);
+        if (threadIdx.x == 0) st->final_hpwl = h;
+        return;
+    }
+    if (st->done) {
+        if (which != 0 && threadIdx.x == 0) st->run_update = 0;
+        return;
+    }
+    if (which == 0 || which == 4) ctrl_after_gather(D, st, sh);
+    if (which == 4) __syncthreads();  // thread 0's state writes, seen by the block
+    if (which == 1 || which == 4) ctrl_end(D, st, sh);
 }
 
 // ------------------------------------------------------------ update ----

@@ -258,7 +258,9 @@ static const char* kPhaseNames[] = {"sort", "wa", "scatter", "dens_final", "dct_
 constexpr int kNumPhases = 14;
 constexpr int kWaPhase = 1;
 constexpr int kGatherPhase = 8;
+constexpr int kCtrlPhase = 9;
 constexpr int kFixPhase = 10;
+constexpr int kCtrlEndPhase = 11;
 
 static void enqueue_phase(Ctx* c, Batch& b, int ph) {
     hipStream_t s = c->stream;
@@ -295,9 +297,13 @@ static void enqueue_iteration(Ctx* c, Batch& b, bool with_fix) {
     launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, c->side);
     hipEventRecord(c->ev_join, c->side);
     for (int ph = 0; ph < kNumPhases; ph++) {
-        if (ph == kWaPhase || (ph == kFixPhase && !with_fix)) continue;
+        if (ph == kWaPhase) continue;
+        if (!with_fix && (ph == kFixPhase || ph == kCtrlEndPhase)) continue;
         if (ph == kGatherPhase) hipStreamWaitEvent(c->stream, c->ev_join, 0);
-        enqueue_phase(c, b, ph);
+        if (ph == kCtrlPhase && !with_fix)
+            launch_ctrl(b.bd, 4, c->stream);  // both control steps, nothing between them
+        else
+            enqueue_phase(c, b, ph);
     }
 }
 
