@@ -82,6 +82,7 @@ struct DesignDev {
     void* g[2][2];             // [parity][axis] combined gradient (or density grad)
     void* pg;                  // per-slot WA gradient (gx, gy) in T [n_slots][2]
     uint32_t* cs_off;          // cell -> slot CSR [n_mov + 1]
+    double* gwl;               // [n_mov][2] WA gradient per cell (its slots summed in FP64)
     uint32_t* cs_idx;          // slots of each movable cell, ascending
     uint32_t* perm[2];         // cells in Morton order of their lower-left bin tile; the
                                // sort of iteration k reads perm[k&1] and writes perm[(k+1)&1]
@@ -156,6 +157,7 @@ void launch_wa(const BatchDev& b, int prec, int total_wa_blocks, int total_hf_bl
 void launch_scatter(const BatchDev& b, int prec, int total_scat_blocks, hipStream_t s);
 void launch_dens_final(const BatchDev& b, int prec, int total_bin_blocks, hipStream_t s);
 void launch_gather(const BatchDev& b, int prec, int total_cell_blocks, int mode, hipStream_t s);
+void launch_wl_cell(const BatchDev& b, int prec, int total_cell_blocks, hipStream_t s);
 void launch_ctrl(const BatchDev& b, int which, hipStream_t s);
 void launch_update(const BatchDev& b, int prec, int total_cell_blocks, hipStream_t s);
 void launch_hpwl(const BatchDev& b, int prec, int total_wa_blocks, int total_hf_blocks,

@@ -629,6 +629,21 @@ __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy)
     gy = sy;
 }
 
+// The WA gradient of every cell (its slots summed, gwl_of), written per cell
+// so the gather reads one contiguous pair; launched right after the WA
+// kernels, beside the density chain.
+template <typename T>
+__global__ __launch_bounds__(256) void k_wl_cell(BatchDev b) {
+    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    if (D.st->done) return;
+    const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    if (i >= D.n_mov) return;
+    double2 v;
+    gwl_of<T>(D, i, v.x, v.y);
+    ((double2*)D.gwl)[i] = v;
+}
+
 // MODE 0: gather; MODE 1: fix (penalty set by ctrl this iteration);
 // MODE 2: evaluation export of the density gradient only.
 template <typename T, bool FIX, int MODE>
@@ -702,8 +717,8 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
             ey_out[i] = gden[1];
         } else if (MODE == 0 && !st->lam_init) {
             // penalty not yet initialised: park the density gradient, norms only
-            double w0, w1;
-            gwl_of<T>(D, i, w0, w1);
+            const double2 wl = ((const double2*)D.gwl)[i];
+            const double w0 = wl.x, w1 = wl.y;
             ((T*)D.g[par][0])[i] = (T)gden[0];
             ((T*)D.g[par][1])[i] = (T)gden[1];
             part[0] = w0 * w0 + w1 * w1;
@@ -712,8 +727,8 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
             const double lam = MODE == 0 ? sched(D.cfg, st->lam) : st->lam;
             double pc = ((const T*)D.pinw)[i] + lam;  // mod.rs:152-156
             pc = pc < 1.0 ? 1.0 : pc;
-            double gwa[2];
-            gwl_of<T>(D, i, gwa[0], gwa[1]);
+            const double2 wl = ((const double2*)D.gwl)[i];
+            const double gwa[2] = {wl.x, wl.y};
 #pragma unroll
             for (int ax = 0; ax < 2; ax++) {
                 const double gw = gwa[ax];
@@ -1061,6 +1076,14 @@ void launch_dens_final(const BatchDev& b, int prec, int bin_blocks, hipStream_t 
         hipLaunchKernelGGL((k_dens_final<double, false>), dim3(bin_blocks), dim3(256), 0, s, b);
     else
         hipLaunchKernelGGL((k_dens_final<float, true>), dim3(bin_blocks), dim3(256), 0, s, b);
+}
+
+void launch_wl_cell(const BatchDev& b, int prec, int cell_blocks, hipStream_t s) {
+    if (cell_blocks <= 0) return;
+    if (prec == EP_PREC_FP64)
+        hipLaunchKernelGGL(k_wl_cell<double>, dim3(cell_blocks), dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL(k_wl_cell<float>, dim3(cell_blocks), dim3(256), 0, s, b);
 }
 
 void launch_gather(const BatchDev& b, int prec, int cell_blocks, int mode, hipStream_t s) {

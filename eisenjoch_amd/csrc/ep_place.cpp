@@ -266,7 +266,10 @@ static void enqueue_phase(Ctx* c, Batch& b, int ph) {
     hipStream_t s = c->stream;
     switch (ph) {
         case 0: launch_tile_sort(b.bd, b.prec, b.cell_blocks, s); break;      // scatter order
-        case 1: launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, s); break;  // mod.rs:110-115
+        case 1:  // mod.rs:110-115
+            launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, s);
+            launch_wl_cell(b.bd, b.prec, b.cell_blocks, s);
+            break;
         case 2:                                                               // mod.rs:118-119
             launch_scatter_pull(b.bd, b.prec, b.tile_blocks, s);
             launch_scatter(b.bd, b.prec, b.scat_blocks, s);
@@ -295,6 +298,7 @@ static void enqueue_iteration(Ctx* c, Batch& b, bool with_fix) {
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->side, c->ev_fork, 0);
     launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, c->side);
+    launch_wl_cell(b.bd, b.prec, b.cell_blocks, c->side);
     hipEventRecord(c->ev_join, c->side);
     for (int ph = 0; ph < kNumPhases; ph++) {
         if (ph == kWaPhase) continue;

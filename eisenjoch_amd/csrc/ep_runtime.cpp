@@ -262,6 +262,7 @@ static size_t plan_arena(Design& D, char* base) {
         for (int ax = 0; ax < 2; ax++) G.g[p][ax] = P(N * T);
     G.pg = P((size_t)G.n_slots * 2 * T);
     G.cs_off = (uint32_t*)P((N + 1) * 4);
+    G.gwl = (double*)P(N * 16);
     G.cs_idx = (uint32_t*)P(D.cs_count * 4);
     G.perm[0] = (uint32_t*)P(N * 4);
     G.perm[1] = (uint32_t*)P(N * 4);
