@@ -43,6 +43,8 @@ __device__ inline void epi_store(const GemmJob& J, const DesignDev& D, int row, 
         ((T*)J.C)[(size_t)row * J.N + col] = (T)v;
     } else if (J.epi == kEpiStoreT) {
         ((T*)J.C)[(size_t)col * J.M + row] = (T)v;
+    } else if (J.epi == kEpiStore2) {
+        ((T*)J.C)[2 * ((size_t)row * J.N + col)] = (T)v;
     } else {
         const T* sx = (const T*)D.spec_x;  // [kx] -> (w, 2(1 - cos w))
         const T* sy = (const T*)D.spec_y;

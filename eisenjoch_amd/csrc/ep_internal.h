@@ -107,7 +107,7 @@ struct DesignDev {
     void* tT;                  // F1 output, transposed [Bx*By]
     void* ehat[2];             // F2 output [By*Bx]
     void* uT[2];               // I1 output, transposed [Bx*By]
-    void* field[2];            // I2 output (Ex, Ey) [By*Bx]
+    void* field;               // I2 output, T [By*Bx][2]: Ex and Ey interleaved
     void* spec_x;              // T[2*Bx]: (w, 2(1-cos w)) per kx
     void* spec_y;              // T[2*By]
     // reductions
@@ -137,7 +137,8 @@ struct GemmJob {
     int32_t pad;
 };
 
-enum Epilogue : int32_t { kEpiStore = 0, kEpiStoreT = 1, kEpiPoisson = 2 };
+// kEpiStore2: row-major into every other element (one component of a pair)
+enum Epilogue : int32_t { kEpiStore = 0, kEpiStoreT = 1, kEpiPoisson = 2, kEpiStore2 = 3 };
 
 // A batch as the kernels see it (device pointers).
 struct BatchDev {

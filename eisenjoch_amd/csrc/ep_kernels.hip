@@ -664,8 +664,8 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
             const typename Vec2<T>::type l = ((const typename Vec2<T>::type*)D.look)[i];
             const double cx = (double)l.x, cy = (double)l.y;
             const BinBox r = bin_box(D, cx, cy, (double)D.hs[i]);
-            const T* E0 = (const T*)D.field[0];
-            const T* E1 = (const T*)D.field[1];
+            typedef typename Vec2<T>::type V;
+            const V* E = (const V*)D.field;
             double fx = 0, fy = 0, wt = 0;
             if (r.bx1 - r.bx0 <= 3 && r.by1 - r.by0 <= 3) {
                 // footprints of up to 3x3 bins (all of them at areas <= 4 and
@@ -677,8 +677,9 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
                     const int by = r.by0 + q / 3, bx = r.bx0 + q % 3;
                     const bool in = by < r.by1 && bx < r.bx1;
                     const size_t idx = in ? (size_t)by * D.Bx + bx : 0;
-                    e0[q] = in ? E0[idx] : T(0);
-                    e1[q] = in ? E1[idx] : T(0);
+                    const V e = E[idx];
+                    e0[q] = in ? e.x : T(0);
+                    e1[q] = in ? e.y : T(0);
                 }
 #pragma unroll
                 for (int q = 0; q < 9; q++) {
@@ -697,9 +698,9 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
                     for (int bx = r.bx0; bx < r.bx1; bx++) {
                         double w = ov_area(r, bx, by);
                         if (w > 0.0) {
-                            size_t idx = (size_t)by * D.Bx + bx;
-                            fx += w * (double)E0[idx];
-                            fy += w * (double)E1[idx];
+                            const V e = E[(size_t)by * D.Bx + bx];
+                            fx += w * (double)e.x;
+                            fy += w * (double)e.y;
                             wt += w;
                         }
                     }

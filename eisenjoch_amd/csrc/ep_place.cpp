@@ -96,6 +96,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         if (!rc) rc = get_transform(c, D.dev.By, cfgs[i]->field_mode, &ty);
         if (rc) return rc;
         const bool f64 = b.prec == EP_PREC_FP64;
+        const size_t tsz = f64 ? 8 : 4;  // element size of the data buffers
         auto cst = [&](GemmJob& J, bool isA, Transform* t, int k) {
             const void* p = f64 ? (const void*)t->d[k] : (const void*)t->f[k];
             if (isA) { J.A = p; J.A_hi = t->hi[k]; J.A_lo = t->lo[k]; J.a_kind = 1; }
@@ -121,9 +122,9 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         dat(i1a, true, D.dev.ehat[0]); cst(i1a, false, tx, 2);
         GemmJob i1b = mk(By, Bx, Bx, kEpiStoreT, D.dev.uT[1], nullptr);  // Ey^ . C3^T (idct_rows)
         dat(i1b, true, D.dev.ehat[1]); cst(i1b, false, tx, 1);
-        GemmJob i2a = mk(By, Bx, By, kEpiStore, D.dev.field[0], nullptr);  // C3 . U (idct_cols)
+        GemmJob i2a = mk(By, Bx, By, kEpiStore2, D.dev.field, nullptr);  // C3 . U (idct_cols)
         cst(i2a, true, ty, 1); dat(i2a, false, D.dev.uT[0]);
-        GemmJob i2b = mk(By, Bx, By, kEpiStore, D.dev.field[1], nullptr);  // S3 . U (idst_cols)
+        GemmJob i2b = mk(By, Bx, By, kEpiStore2, (char*)D.dev.field + tsz, nullptr);  // S3 . U (idst_cols)
         cst(i2b, true, ty, 2); dat(i2b, false, D.dev.uT[1]);
         const int eng = engine_for(c, b.prec, Bx, By);
         jobs[0][eng].push_back(f1);
@@ -885,8 +886,13 @@ int ep_eval_field(ep_ctx* h, uint32_t id, const double* dens, double* ex, double
             if (b->njobs2[p][e] > 0)
                 launch_gemm_pass(b->jobs2[p][e], b->njobs2[p][e], b->tiles2[p][e], b->d_dev, b->prec, e, c->stream);
     HIPCHK(c, hipGetLastError());
-    if ((rc = get_T(c, ex, D->dev.field[0], NB, D->prec))) return rc;
-    return get_T(c, ey, D->dev.field[1], NB, D->prec);
+    std::vector<double> xy(2 * NB);
+    if ((rc = get_T(c, xy.data(), D->dev.field, 2 * NB, D->prec))) return rc;
+    for (size_t k = 0; k < NB; k++) {
+        ex[k] = xy[2 * k];
+        ey[k] = xy[2 * k + 1];
+    }
+    return EP_OK;
 }
 
 int ep_eval_density_grad(ep_ctx* h, uint32_t id, const double* x, const double* y, const double* ex,
@@ -901,8 +907,7 @@ int ep_eval_density_grad(ep_ctx* h, uint32_t id, const double* x, const double* 
     const size_t N = (size_t)D->dev.n_mov, NB = (size_t)D->dev.Bx * D->dev.By;
     if (!N) return EP_OK;
     if ((rc = put_xy(c, D->dev.look, x, y, N, D->prec))) return rc;
-    if ((rc = put_T(c, D->dev.field[0], ex, NB, D->prec))) return rc;
-    if ((rc = put_T(c, D->dev.field[1], ey, NB, D->prec))) return rc;
+    if ((rc = put_xy(c, D->dev.field, ex, ey, NB, D->prec))) return rc;
     double* dbuf = nullptr;
     HIPCHK(c, hipMalloc(&dbuf, N * 16));
     launch_gather_export(b->bd, b->prec, b->cell_blocks, dbuf, dbuf + N, c->stream);

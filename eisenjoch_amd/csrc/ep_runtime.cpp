@@ -285,7 +285,7 @@ static size_t plan_arena(Design& D, char* base) {
     G.tT = P(B * T);
     for (int ax = 0; ax < 2; ax++) G.ehat[ax] = P(B * T);
     for (int ax = 0; ax < 2; ax++) G.uT[ax] = P(B * T);
-    for (int ax = 0; ax < 2; ax++) G.field[ax] = P(B * T);
+    G.field = P(B * 2 * T);
     G.spec_x = P((size_t)2 * G.Bx * T);
     G.spec_y = P((size_t)2 * G.By * T);
     G.part_cell = (double*)P((size_t)G.n_cell_blocks * kCellParts * 8);
