@@ -90,7 +90,7 @@ struct DesignDev {
     void* stage_xy;            // T [n_mov][2]: look-ahead position of perm[k&1][j], by j
     float* stage_hs;           // half side of perm[k&1][j], by j
     void* sorted_xy;           // T [n_mov][2]: look-ahead positions in the new sort order
-    float* sorted_hs;          // half sides in the new sort order
+    float* sorted_hs[2];       // half sides in the order of perm[0] / perm[1]
     uint32_t* tcount;          // [n_keys] cells per tile key (zero between sorts)
     uint32_t* tstart;          // [n_keys + 1] first position of each key in perm
     uint32_t* tcur;            // [n_keys] fill cursors

@@ -436,8 +436,9 @@ __device__ inline uint32_t wave_agg_add(uint32_t* ctr, uint32_t key, bool valid)
 // Counting sort of the cells by tile key, every iteration: count, scan, fill.
 // Both passes walk the cells in the previous iteration's order, in which
 // neighbours mostly share a tile, so a wave sees few distinct keys.  The
-// count pass gathers each cell's position and half side once and stages
-// them by position, so the fill pass reads only contiguous arrays.
+// count pass gathers each cell's position once (the half sides are kept in
+// the order of each permutation by the fill that wrote it) and stages both by
+// position, so the fill pass reads only contiguous arrays.
 template <typename T>
 __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
     const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
@@ -447,10 +448,11 @@ __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
     const bool valid = k < D.n_mov;
     uint32_t key = 0u;
     if (valid) {
-        const uint32_t i = D.perm[D.st->it & 1][k];
+        const int par = D.st->it & 1;
+        const uint32_t i = D.perm[par][k];
         const typename Vec2<T>::type l = ((const typename Vec2<T>::type*)D.look)[i];
         const T x = l.x, y = l.y;
-        const float hs = D.hs[i];
+        const float hs = D.sorted_hs[par][k];  // hs[i], already in this order
         key = tile_key<T>(D, x, y, hs);
         D.keys[k] = key;
         T* sxy = (T*)D.stage_xy;
@@ -512,7 +514,7 @@ __global__ __launch_bounds__(256) void k_tile_fill(BatchDev b) {
         T* sxy = (T*)D.sorted_xy;
         sxy[2 * (size_t)pos] = gxy[2 * (size_t)k];
         sxy[2 * (size_t)pos + 1] = gxy[2 * (size_t)k + 1];
-        D.sorted_hs[pos] = D.stage_hs[k];
+        D.sorted_hs[par ^ 1][pos] = D.stage_hs[k];
     }
 }
 
@@ -539,6 +541,7 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
     for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) win[k] = Acc(0);
     __syncthreads();
     const T* SXY = (const T*)D.sorted_xy;
+    const float* SHS = D.sorted_hs[(D.st->it & 1) ^ 1];
     // ranges: q = 0 the 4x4 block of tiles (Morton keys 16*m .. 16*m+15),
     // q = 1..9 the halo tiles (left column and bottom row, corner included)
     for (int q = 0; q < 10; q++) {
@@ -560,7 +563,7 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
         }
         for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
             const BinBox r = bin_box(D, (double)SXY[2 * (size_t)k], (double)SXY[2 * (size_t)k + 1],
-                                     (double)D.sorted_hs[k]);
+                                     (double)SHS[k]);
             const int bx0 = max(r.bx0, x0), bx1 = min(r.bx1, x0 + 32);
             const int by0 = max(r.by0, y0), by1 = min(r.by1, y0 + 32);
             for (int by = by0; by < by1; by++)
@@ -973,6 +976,7 @@ __global__ __launch_bounds__(256) void k_reset(BatchDev b) {
     const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
     D.perm[0][i] = (uint32_t)i;
+    D.sorted_hs[0][i] = D.hs[i];
     T lk[2];
 #pragma unroll
     for (int ax = 0; ax < 2; ax++) {

@@ -270,7 +270,8 @@ static size_t plan_arena(Design& D, char* base) {
     G.stage_xy = P(N * 2 * T);
     G.stage_hs = (float*)P(N * 4);
     G.sorted_xy = P(N * 2 * T);
-    G.sorted_hs = (float*)P(N * 4);
+    G.sorted_hs[0] = (float*)P(N * 4);
+    G.sorted_hs[1] = (float*)P(N * 4);
     G.tcount = (uint32_t*)P((size_t)G.n_keys * 4);
     G.tstart = (uint32_t*)P(((size_t)G.n_keys + 1) * 4);
     G.tcur = (uint32_t*)P((size_t)G.n_keys * 4);
@@ -525,6 +526,8 @@ static int fill_arena(Ctx* c, Design* D, const ep_design* d, const NetLayout& L)
         for (size_t i = 0; i < N; i++) id[i] = (uint32_t)i;
         if (N) HIPCHK(c, hipMemcpy(D->dev.perm[0], id.data(), N * 4, hipMemcpyHostToDevice));
         if (N) HIPCHK(c, hipMemcpy(D->dev.perm[1], id.data(), N * 4, hipMemcpyHostToDevice));
+        for (int p = 0; p < 2; p++)
+            if (N) HIPCHK(c, hipMemcpy(D->dev.sorted_hs[p], hs.data(), N * 4, hipMemcpyHostToDevice));
     }
     if (!L.cs_idx.empty())
         HIPCHK(c, hipMemcpy(D->dev.cs_idx, L.cs_idx.data(), L.cs_idx.size() * 4, hipMemcpyHostToDevice));
