@@ -140,6 +140,7 @@ SIGNATURES = {
     "ep_calibrate_copy": (C.c_int, [_p, C.c_uint64]),
     "ep_autotune": (C.c_int, [_p, C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32, C.c_char_p]),
     "ep_tune_table": (C.c_int, [_p, C.POINTER(C.c_int32), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "ep_tune_set": (C.c_int, [_p, C.c_int32, C.c_int32, C.c_int32]),
     "ep_legalize": (C.c_int, [C.c_int32, _p, _p, _p, C.c_int32, _p, _p, _p, _p, _p, _p]),
     "ep_generate": (C.c_int, [C.POINTER(EpGenParams), C.POINTER(C.POINTER(EpGenDesign))]),
     "ep_gen_free": (None, [C.POINTER(EpGenDesign)]),

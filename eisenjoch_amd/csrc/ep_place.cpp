@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -546,12 +547,55 @@ int ep_iter_log(ep_ctx* h, uint32_t id, ep_iter_rec* recs, uint32_t cap, uint32_
 
 // ---------------------------------------------------------- autotuner ----
 
-// Times every eligible transform engine on one resident design of each
-// distinct bin shape among `ids` and keeps the fastest in the context's
-// table.  With a cache path, shapes already in the file are taken from it
-// and new results are appended.  Engines differ only in FP32 rounding order,
-// and the table is per shape, so a design's results do not depend on the
-// batch it runs in.
+// Chooses the transform engine of every distinct bin shape among `ids` by
+// timing the four transform passes of the whole batch `ids` (the launches
+// the placement loop makes, so each candidate is measured under the
+// occupancy the other shapes' tiles impose), one shape at a time from the
+// largest, each candidate with the other shapes at their current choice;
+// the fastest is kept in the context's table.  With a cache path, shapes
+// already in the file are taken from it and new results are appended.
+// Engines differ only in FP32 rounding order, and the table is per shape,
+// so a design's results do not depend on the batch it runs in.
+static int time_transforms(Ctx* c, const std::vector<Design*>& ds, uint32_t reps, float* ms_out) {
+    ep_config cfg;
+    ep_config_default(&cfg);
+    cfg.field_mode = EP_FIELD_IDXST;
+    release_batch_cache(c);
+    Batch b;
+    std::vector<const ep_config*> cf(ds.size(), &cfg);
+    int rc = batch_build_cfgs(c, ds, cf, b);
+    if (rc) return rc;
+    // live design states so the transform kernels do not skip them
+    for (Design* D : ds) {
+        DevState st;
+        memset(&st, 0, sizeof(st));
+        st.a = 1.0;
+        c->state_host[D->state_idx] = st;
+        hipError_t er = hipMemcpyAsync(D->dev.st, &c->state_host[D->state_idx], sizeof(DevState),
+                                       hipMemcpyHostToDevice, c->stream);
+        if (er != hipSuccess) { batch_release(b); return fail(c, EP_ERR_DEVICE, hipGetErrorString(er)); }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t er = hipEventCreate(&e0);
+    if (er == hipSuccess) er = hipEventCreate(&e1);
+    if (er == hipSuccess) {
+        for (int p = 0; p < 4; p++) enqueue_gemm(c, b, p);  // warm
+        hipEventRecord(e0, c->stream);
+        for (uint32_t r = 0; r < reps; r++)
+            for (int p = 0; p < 4; p++) enqueue_gemm(c, b, p);
+        hipEventRecord(e1, c->stream);
+        er = hipEventSynchronize(e1);
+    }
+    float ms = 0.f;
+    if (er == hipSuccess) er = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    batch_release(b);
+    if (er != hipSuccess) return fail(c, EP_ERR_DEVICE, hipGetErrorString(er));
+    *ms_out = ms;
+    return EP_OK;
+}
+
 extern "C" int ep_autotune(ep_ctx* h, const uint32_t* ids, uint32_t n, uint32_t reps,
                            const char* cache_path) {
     Ctx* c = (Ctx*)h;
@@ -570,60 +614,40 @@ extern "C" int ep_autotune(ep_ctx* h, const uint32_t* ids, uint32_t n, uint32_t 
         }
     }
     if (reps == 0) reps = 5;
-    std::map<std::pair<int, int>, Design*> shapes;
+    std::vector<Design*> ds;
+    std::set<std::pair<int, int>> shapes;
     for (uint32_t i = 0; i < n; i++) {
         Design* D = find_design(c, ids[i]);
         if (!D) return fail(c, EP_ERR_NOT_FOUND, "unknown design id");
-        if (D->prec != EP_PREC_FAST || D->dev.Bx % 64 || D->dev.By % 64) continue;
-        shapes.emplace(std::make_pair(D->dev.Bx, D->dev.By), D);
+        if (D->prec != EP_PREC_FAST) continue;
+        if (std::find(ds.begin(), ds.end(), D) == ds.end()) ds.push_back(D);
+        if (D->dev.Bx % 64 == 0 && D->dev.By % 64 == 0) shapes.insert({D->dev.Bx, D->dev.By});
     }
-    ep_config cfg;
-    ep_config_default(&cfg);
-    cfg.field_mode = EP_FIELD_IDXST;
+    std::vector<std::pair<int, int>> todo;
+    for (auto& sh : shapes) {
+        auto hit = cached.find(sh);
+        if (hit != cached.end()) c->tune[sh] = hit->second;
+        else todo.push_back(sh);
+    }
+    // largest shapes first: they bound a batched pass
+    std::sort(todo.begin(), todo.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+        return (int64_t)a.first * a.second > (int64_t)b.first * b.second;
+    });
+    for (auto& sh : todo)
+        if (!c->tune.count(sh)) c->tune[sh] = 0;
     std::vector<std::pair<std::pair<int, int>, int>> fresh;
-    for (auto& kv : shapes) {
-        auto hit = cached.find(kv.first);
-        if (hit != cached.end()) {
-            c->tune[kv.first] = hit->second;
-            continue;
-        }
-        double best = 1e300;
-        int best_e = 0;
-        for (int e : {3, 0, 2, 1}) {
-            c->tune[kv.first] = e;
-            release_batch_cache(c);
-            Batch b;
-            std::vector<Design*> ds{kv.second};
-            std::vector<const ep_config*> cf{&cfg};
-            int rc = batch_build_cfgs(c, ds, cf, b);
-            if (rc) return rc;
-            // a live design state so the transform kernels do not skip it
-            DevState st;
-            memset(&st, 0, sizeof(st));
-            st.a = 1.0;
-            c->state_host[kv.second->state_idx] = st;
-            hipError_t er = hipMemcpyAsync(kv.second->dev.st, &c->state_host[kv.second->state_idx],
-                                           sizeof(DevState), hipMemcpyHostToDevice, c->stream);
-            hipEvent_t e0, e1;
-            if (er == hipSuccess) er = hipEventCreate(&e0);
-            if (er == hipSuccess) er = hipEventCreate(&e1);
-            if (er != hipSuccess) { batch_release(b); return fail(c, EP_ERR_DEVICE, hipGetErrorString(er)); }
-            for (int p = 0; p < 4; p++) enqueue_gemm(c, b, p);  // warm
-            hipEventRecord(e0, c->stream);
-            for (uint32_t r = 0; r < reps; r++)
-                for (int p = 0; p < 4; p++) enqueue_gemm(c, b, p);
-            hipEventRecord(e1, c->stream);
-            er = hipEventSynchronize(e1);
+    for (auto& sh : todo) {
+        float best = 1e30f;
+        int best_e = c->tune[sh];
+        for (int e : {0, 3, 2, 1}) {
+            c->tune[sh] = e;
             float ms = 0.f;
-            if (er == hipSuccess) er = hipEventElapsedTime(&ms, e0, e1);
-            hipEventDestroy(e0);
-            hipEventDestroy(e1);
-            batch_release(b);
-            if (er != hipSuccess) return fail(c, EP_ERR_DEVICE, hipGetErrorString(er));
+            int rc = time_transforms(c, ds, reps, &ms);
+            if (rc) return rc;
             if (ms < best) { best = ms; best_e = e; }
         }
-        c->tune[kv.first] = best_e;
-        fresh.push_back({kv.first, best_e});
+        c->tune[sh] = best_e;
+        fresh.push_back({sh, best_e});
     }
     release_batch_cache(c);
     if (cache_path && !fresh.empty()) {
@@ -651,6 +675,21 @@ extern "C" int ep_tune_table(ep_ctx* h, int32_t* out, uint32_t cap, uint32_t* n)
         k++;
     }
     *n = k;
+    return EP_OK;
+}
+
+// Pins the transform engine of one bin shape (engine < 0 removes the entry).
+extern "C" int ep_tune_set(ep_ctx* h, int32_t bx, int32_t by, int32_t engine) {
+    Ctx* c = (Ctx*)h;
+    if (!c) return fail(c, EP_ERR_INVALID_ARG, "null context");
+    if (engine >= kEngines) return fail(c, EP_ERR_INVALID_ARG, "unknown transform engine");
+    if (engine >= 0 && engine != 1 && (bx % 64 || by % 64))
+        return fail(c, EP_ERR_INVALID_ARG, "MFMA engines need bins that are multiples of 64");
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    if (c->server_running) return fail(c, EP_ERR_INVALID_ARG, "context is serving; stop it first");
+    if (engine < 0) c->tune.erase({bx, by});
+    else c->tune[{bx, by}] = engine;
+    release_batch_cache(c);  // cached batches hold transform jobs split by engine
     return EP_OK;
 }
 

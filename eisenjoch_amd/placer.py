@@ -400,20 +400,29 @@ class Device:
                                             C.byref(k)), self._h, "ep_profile_iteration")
         return {out[i].name.decode(): out[i].avg_ms for i in range(k.value)}
 
+    ENGINES = {0: "mfma-k32", 1: "fma", 2: "mfma-splitk", 3: "mfma-k64"}
+
     def autotune(self, ids: Sequence[int], reps: int = 5, cache_path: Optional[str] = None) -> dict:
-        """Pick the fastest transform engine per bin shape (ep_autotune)."""
+        """Pick the fastest transform engine per bin shape (ep_autotune);
+        returns the tuning table {"BxxBy": engine name}."""
         n = len(ids)
         arr = (C.c_uint32 * n)(*ids)
         path = cache_path.encode() if cache_path else None
         _check(L.lib().ep_autotune(self._h, arr, n, reps, path), self._h, "ep_autotune")
+        return self.tune_table()
+
+    def tune_table(self) -> dict:
         out = (C.c_int32 * 3 * 64)()
         k = C.c_uint32()
         _check(L.lib().ep_tune_table(self._h, C.cast(out, C.POINTER(C.c_int32)), 64, C.byref(k)),
                self._h, "ep_tune_table")
-        names = {0: "mfma-staged", 1: "fma", 2: "mfma-splitk"}
         flat = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_int32)), shape=(64 * 3,))
-        return {f"{flat[3 * i]}x{flat[3 * i + 1]}": names.get(int(flat[3 * i + 2]), "?")
+        return {f"{flat[3 * i]}x{flat[3 * i + 1]}": self.ENGINES.get(int(flat[3 * i + 2]), "?")
                 for i in range(min(k.value, 64))}
+
+    def tune_set(self, bx: int, by: int, engine: int):
+        """Pin the transform engine of one bin shape (ep_tune_set; < 0 clears)."""
+        _check(L.lib().ep_tune_set(self._h, bx, by, engine), self._h, "ep_tune_set")
 
     # -- single-kernel evaluation --
     def eval_config(self, cfg: ElectroPlaceCfg):

@@ -197,15 +197,21 @@ int ep_profile_iteration(ep_ctx* ctx, const uint32_t* ids, uint32_t n, const ep_
                          uint32_t warm, uint32_t reps, ep_phase_time* out, uint32_t cap,
                          uint32_t* n_out);
 
-/* Autotuner (no counterpart in the reference): times every eligible
+/* Autotuner (no counterpart in the reference): for each distinct bin shape
+ * among the FAST designs `ids`, largest first, times every eligible
  * transform engine (engine 0: MFMA with LDS staging in K stages of 32;
- * 1: FP32 FMA; 2: MFMA split-K; 3: MFMA with K stages of 64) on one
- * resident FAST design of each distinct bin shape among `ids`, `reps` times,
- * and keeps the fastest per shape for every later batch.  With a cache path,
- * shapes found in the file are reused and new results appended. */
+ * 1: FP32 FMA; 2: MFMA split-K; 3: MFMA with K stages of 64) on the four
+ * transform passes of the whole batch `ids`, `reps` times, with the other
+ * shapes at their current choice, and keeps the fastest per shape for every
+ * later batch.  With a cache path, shapes found in the file are reused and
+ * new results appended. */
 int ep_autotune(ep_ctx* ctx, const uint32_t* ids, uint32_t n, uint32_t reps, const char* cache_path);
 /* The tuning table as (Bx, By, engine) triples; *n receives the entry count. */
 int ep_tune_table(ep_ctx* ctx, int32_t* out, uint32_t cap, uint32_t* n);
+/* Pins the transform engine of one bin shape in the tuning table (engine < 0
+ * removes the entry; the MFMA engines 0, 2, 3 need bins that are multiples of
+ * 64).  No counterpart in the reference. */
+int ep_tune_set(ep_ctx* ctx, int32_t bx, int32_t by, int32_t engine);
 
 /* Copies `bytes` device-to-device once with the runtime's plain copy kernel
  * (no counterpart in the reference): a transfer of known size against which
