@@ -252,7 +252,8 @@ KERNEL_GROUPS = {
         "kernels": {"ep::k_tile_count<float>": 1, "ep::k_tile_scan(": 1, "ep::k_tile_fill<float>": 1,
                     "ep::k_scatter_pull<float, true>": 1}},
     "wa": {"phases": ["wa"], "bound": "hbm", "work": "wa",
-           "kernels": {"ep::k_wa<float, true, false>": 1}},
+           "kernels": {"ep::k_wa<float, true, false>": 1, "ep::k_wl_cell<float>": 1},
+           "note": "second stream: overlaps the tile sort, scatter and transforms"},
     "dct (4 transform passes, BF16 hi+lo MFMA)": {
         "phases": ["dct_f1", "dct_f2", "dct_i1", "dct_i2"], "bound": "mfma", "work": "dct_flops",
         "kernels": {"ep::k_gemm_mfma(": 4}},
@@ -427,6 +428,8 @@ def main():
                            "traffic": traffic}
     out["roofline"]["algorithmic_per_launch"] = work[g["work"]]
     out["roofline"]["launch_us"] = round(ms * 1e3, 2)
+    if "note" in g:
+        out["roofline"]["note"] = g["note"]
     out["phase_ms_per_iter"] = {k: round(v, 4) for k, v in prof.items()}
     out["kernel_rooflines"] = {}
     for name, (gms, gg) in groups.items():

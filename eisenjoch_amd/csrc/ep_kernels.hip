@@ -465,17 +465,31 @@ __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
 
 // One block per design: tstart = exclusive scan of the counts (with the
 // total at n_keys), cursors = tstart, counts cleared for the next sort.
-// Each thread owns a contiguous run of keys; the run totals are scanned
-// with wave shuffles and one LDS step across the four waves.
+// Each thread owns a contiguous run of keys (16-byte vectors when the key
+// count is a multiple of 1,024, as every power of four from 1,024 up is);
+// the run totals are scanned with wave shuffles and one LDS step across the
+// four waves.  The three arrays are distinct buffers (restrict), so a run's
+// loads are all issued before its stores.
 __global__ __launch_bounds__(256) void k_tile_scan(BatchDev b) {
     __shared__ uint32_t wsum[4];
     const DesignDev& D = b.d[blockIdx.x];
     if (D.st->done) return;
+    uint32_t* __restrict__ cnt = D.tcount;
+    uint32_t* __restrict__ ts = D.tstart;
+    uint32_t* __restrict__ tc = D.tcur;
     const int n = D.n_keys;
+    const bool vec = (n & 1023) == 0;
     const int per = (n + 255) / 256;
     const int lo = min((int)threadIdx.x * per, n), hi = min(lo + per, n);
     uint32_t sum = 0;
-    for (int k = lo; k < hi; k++) sum += D.tcount[k];
+    if (vec) {
+        for (int k = lo; k < hi; k += 4) {
+            const uint4 v = *(const uint4*)&cnt[k];
+            sum += (v.x + v.y) + (v.z + v.w);
+        }
+    } else {
+        for (int k = lo; k < hi; k++) sum += cnt[k];
+    }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t v = sum;  // inclusive scan of the run totals within the wave
     for (int o = 1; o < 64; o <<= 1) {
@@ -486,14 +500,28 @@ __global__ __launch_bounds__(256) void k_tile_scan(BatchDev b) {
     __syncthreads();
     uint32_t base = 0;
     for (int q = 0; q < w; q++) base += wsum[q];
-    if (threadIdx.x == 255) D.tstart[n] = base + v;
+    if (threadIdx.x == 255) ts[n] = base + v;
     uint32_t run = base + v - sum;
-    for (int k = lo; k < hi; k++) {
-        const uint32_t c = D.tcount[k];
-        D.tstart[k] = run;
-        D.tcur[k] = run;
-        D.tcount[k] = 0u;
-        run += c;
+    if (vec) {
+        for (int k = lo; k < hi; k += 4) {
+            const uint4 c = *(const uint4*)&cnt[k];
+            uint4 st;
+            st.x = run; run += c.x;
+            st.y = run; run += c.y;
+            st.z = run; run += c.z;
+            st.w = run; run += c.w;
+            *(uint4*)&ts[k] = st;
+            *(uint4*)&tc[k] = st;
+            *(uint4*)&cnt[k] = make_uint4(0u, 0u, 0u, 0u);
+        }
+    } else {
+        for (int k = lo; k < hi; k++) {
+            const uint32_t c = cnt[k];
+            ts[k] = run;
+            tc[k] = run;
+            cnt[k] = 0u;
+            run += c;
+        }
     }
 }
 
