@@ -75,6 +75,7 @@ def single(args):
     cfg = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
     dev = ep.Device(0)
     ids = {p: dev.upload(d, p) for p in ("fp64", "fast")}
+    tune = dev.autotune([ids["fast"]], reps=5)  # engine for this bin shape, before timing
     res = {}
     for p, i in ids.items():
         dev.place(i, cfg, positions=False)  # warm (transforms, graph)
@@ -91,7 +92,7 @@ def single(args):
         "value": round(1.0 / t32, 3), "unit": "designs/s", "n_gpus": 1, "steps": max(1, args.steps),
         "higher_is_better": True,
         "config": {"workload": d.name, "cells": d.n_mov, "nets": d.n_nets, "pins": d.n_pins,
-                   "bins": [d.Bx, d.By], "tiles": [d.W, d.H]},
+                   "bins": [d.Bx, d.By], "tiles": [d.W, d.H], "transform_engines": tune},
         "fast": {"seconds": round(t32, 4), "iters": r32.iters, "converged": r32.converged,
                  "overflow": r32.overflow, "hpwl": r32.hpwl, "ms_per_iter": round(t32 / r32.iters * 1e3, 4)},
         "plain_fp64": {"seconds": round(t64, 4), "iters": r64.iters, "converged": r64.converged,
