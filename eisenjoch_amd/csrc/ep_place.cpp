@@ -446,7 +446,7 @@ int free_design_locked(Ctx* c, uint32_t id) {
         for (auto* D : c->cache->ds)
             if (D->id == id) { release_batch_cache(c); break; }
     hipStreamSynchronize(c->stream);
-    hipFree(it->second->arena);
+    arena_release(c, it->second->arena, it->second->bytes);
     c->used -= it->second->bytes;
     c->free_states.push_back(it->second->state_idx);
     c->designs.erase(it);

@@ -140,15 +140,6 @@ int get_transform(Ctx* c, int n, int mode, Transform** out) {
 // net; their pins take the slots after the wave slots.  Nets with fewer
 // than two pins are skipped, as common.rs:242 does.  The cell -> slot CSR
 // lists, for every movable cell, the slots of its pins in ascending order.
-struct NetLayout {
-    std::vector<uint32_t> slots;
-    int n_buckets = kMaxBuckets;
-    int bucket_wave_start[kMaxBuckets + 1] = {0};
-    int bucket_group[kMaxBuckets] = {0};
-    std::vector<uint32_t> hf_off, hf_pins;
-    std::vector<uint32_t> cs_off, cs_idx;
-};
-
 static void build_layout(const ep_design* d, NetLayout& L) {
     std::vector<std::vector<std::pair<uint32_t, int>>> nets(kMaxBuckets);
     L.hf_off.assign(1, 0);
@@ -296,24 +287,6 @@ static size_t plan_arena(Design& D, char* base) {
     return a.off;
 }
 
-template <typename T>
-static std::vector<T> conv(const double* p, size_t n) {
-    std::vector<T> v(n);
-    for (size_t i = 0; i < n; i++) v[i] = (T)p[i];
-    return v;
-}
-
-static int upload_T(Ctx* c, void* dst, const std::vector<double>& src, int prec) {
-    if (src.empty()) return EP_OK;
-    if (prec == EP_PREC_FP64) {
-        HIPCHK(c, hipMemcpy(dst, src.data(), src.size() * 8, hipMemcpyHostToDevice));
-    } else {
-        std::vector<float> f = conv<float>(src.data(), src.size());
-        HIPCHK(c, hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice));
-    }
-    return EP_OK;
-}
-
 static void fill_desc_sizes(Design& D, const ep_design* d, const NetLayout& L) {
     DesignDev& G = D.dev;
     memset(&G, 0, sizeof(G));
@@ -435,6 +408,7 @@ void ep_destroy(ep_ctx* h) {
         hipFree(kv.second->arena);
     }
     c->designs.clear();
+    arena_pool_clear(c);
     for (auto& kv : c->transforms) hipFree(kv.second->mem);
     c->transforms.clear();
     release_batch_cache(c);
@@ -470,30 +444,49 @@ uint64_t ep_design_bytes(const ep_design* d, int32_t prec) {
 int ep_design_upload(ep_ctx* h, const ep_design* d, int32_t prec, uint32_t* id_out) {
     Ctx* c = (Ctx*)h;
     if (!c || !id_out) return fail(c, EP_ERR_INVALID_ARG, "null argument");
-    int rc = validate_design(c, d);
-    if (rc) return rc;
     if (prec != EP_PREC_FP64 && prec != EP_PREC_FAST)
         return fail(c, EP_ERR_INVALID_ARG, "unknown precision");
+    Prepared P;
+    int rc = prepare_design(c, d, prec, kDefaultLogCap, P);  // host work outside the lock
+    if (rc) return rc;
     std::lock_guard<std::recursive_mutex> g(c->mu);
-    return upload_locked(c, d, prec, kDefaultLogCap, id_out);
+    rc = commit_design(c, P, id_out);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // P's host vectors are read until here
+    return EP_OK;
 }
 
 }  // extern "C"
 
 namespace ep {
 
-static int fill_arena(Ctx* c, Design* D, const ep_design* d, const NetLayout& L) {
-    const int prec = D->prec;
-    const size_t N = (size_t)d->n_mov;
-    int rc;
-    std::vector<double> tmp(d->x0, d->x0 + N);
-    if ((rc = upload_T(c, D->dev.x0[0], tmp, prec))) return rc;
-    tmp.assign(d->y0, d->y0 + N);
-    if ((rc = upload_T(c, D->dev.x0[1], tmp, prec))) return rc;
-    tmp.assign(d->fixed_x, d->fixed_x + d->n_fixed);
-    if ((rc = upload_T(c, D->dev.fixed[0], tmp, prec))) return rc;
-    tmp.assign(d->fixed_y, d->fixed_y + d->n_fixed);
-    if ((rc = upload_T(c, D->dev.fixed[1], tmp, prec))) return rc;
+template <typename T>
+static void to_T(std::vector<char>& out, const double* p, size_t n) {
+    out.resize(n * sizeof(T));
+    T* o = (T*)out.data();
+    for (size_t i = 0; i < n; i++) o[i] = (T)p[i];
+}
+
+static void to_prec(std::vector<char>& out, const double* p, size_t n, int prec) {
+    if (prec == EP_PREC_FP64) to_T<double>(out, p, n);
+    else to_T<float>(out, p, n);
+}
+
+int prepare_design(Ctx* c, const ep_design* d, int prec, int log_cap, Prepared& P) {
+    int rc = validate_design(c, d);
+    if (rc) return rc;
+    build_layout(d, P.L);
+    P.D.reset(new Design());
+    Design& D = *P.D;
+    D.prec = prec == EP_PREC_FP64 ? EP_PREC_FP64 : EP_PREC_FAST;
+    D.log_cap = log_cap;
+    fill_desc_sizes(D, d, P.L);
+    P.bytes = plan_arena(D, nullptr);
+    const size_t N = (size_t)d->n_mov, F = (size_t)d->n_fixed;
+    to_prec(P.x0[0], d->x0, N, D.prec);
+    to_prec(P.x0[1], d->y0, N, D.prec);
+    to_prec(P.fixed[0], d->fixed_x, F, D.prec);
+    to_prec(P.fixed[1], d->fixed_y, F, D.prec);
     // pin weights, common.rs:381-407 (or the caller's)
     std::vector<double> w(N, 0.0);
     if (d->pinw) {
@@ -509,76 +502,122 @@ static int fill_arena(Ctx* c, Design* D, const ep_design* d, const NetLayout& L)
             }
         }
     }
-    if ((rc = upload_T(c, D->dev.pinw, w, prec))) return rc;
-    std::vector<float> hs(N);
+    to_prec(P.pinw, w.data(), N, D.prec);
+    P.hs.resize(N);
     for (size_t i = 0; i < N; i++)
-        hs[i] = (float)(0.5 * std::sqrt((double)(d->area ? d->area[i] : 1.0f)));
-    if (N) HIPCHK(c, hipMemcpy(D->dev.hs, hs.data(), N * 4, hipMemcpyHostToDevice));
-    if (!L.slots.empty())
-        HIPCHK(c, hipMemcpy(D->dev.slots, L.slots.data(), L.slots.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(D->dev.hf_off, L.hf_off.data(), L.hf_off.size() * 4, hipMemcpyHostToDevice));
-    if (!L.hf_pins.empty())
-        HIPCHK(c, hipMemcpy(D->dev.hf_pins, L.hf_pins.data(), L.hf_pins.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(D->dev.cs_off, L.cs_off.data(), L.cs_off.size() * 4, hipMemcpyHostToDevice));
-    {
-        // scatter order: any permutation of the cells is valid; start with the identity
-        std::vector<uint32_t> id(N);
-        for (size_t i = 0; i < N; i++) id[i] = (uint32_t)i;
-        if (N) HIPCHK(c, hipMemcpy(D->dev.perm[0], id.data(), N * 4, hipMemcpyHostToDevice));
-        if (N) HIPCHK(c, hipMemcpy(D->dev.perm[1], id.data(), N * 4, hipMemcpyHostToDevice));
-        for (int p = 0; p < 2; p++)
-            if (N) HIPCHK(c, hipMemcpy(D->dev.sorted_hs[p], hs.data(), N * 4, hipMemcpyHostToDevice));
-    }
-    if (!L.cs_idx.empty())
-        HIPCHK(c, hipMemcpy(D->dev.cs_idx, L.cs_idx.data(), L.cs_idx.size() * 4, hipMemcpyHostToDevice));
+        P.hs[i] = (float)(0.5 * std::sqrt((double)(d->area ? d->area[i] : 1.0f)));
+    // scatter order: any permutation of the cells is valid; start with the identity
+    P.ident.resize(N);
+    for (size_t i = 0; i < N; i++) P.ident[i] = (uint32_t)i;
     const double pi = 3.14159265358979323846;
-    std::vector<double> sx(2 * (size_t)D->dev.Bx), sy(2 * (size_t)D->dev.By);
-    for (int k = 0; k < D->dev.Bx; k++) {  // density.rs:139-140
-        double w0 = pi * k / (double)D->dev.Bx;
-        sx[2 * k] = w0;
-        sx[2 * k + 1] = 2.0 * (1.0 - std::cos(w0));
+    for (int ax = 0; ax < 2; ax++) {  // density.rs:135-140
+        const int B = ax == 0 ? D.dev.Bx : D.dev.By;
+        std::vector<double> sp(2 * (size_t)B);
+        for (int k = 0; k < B; k++) {
+            double w0 = pi * k / (double)B;
+            sp[2 * k] = w0;
+            sp[2 * k + 1] = 2.0 * (1.0 - std::cos(w0));
+        }
+        to_prec(P.spec[ax], sp.data(), sp.size(), D.prec);
     }
-    for (int k = 0; k < D->dev.By; k++) {  // density.rs:135-136
-        double w0 = pi * k / (double)D->dev.By;
-        sy[2 * k] = w0;
-        sy[2 * k + 1] = 2.0 * (1.0 - std::cos(w0));
+    return EP_OK;
+}
+
+static int copy_in(Ctx* c, void* dst, const void* src, size_t n) {
+    if (n) HIPCHK(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
+    return EP_OK;
+}
+
+int arena_alloc(Ctx* c, size_t bytes, void** out, size_t* got) {
+    // a pooled arena of at least this size and at most a quarter (plus 1 MiB) larger
+    auto it = c->arena_pool.lower_bound(bytes);
+    if (it != c->arena_pool.end() && it->first <= bytes + bytes / 4 + (1u << 20)) {
+        *out = it->second;
+        *got = it->first;
+        c->pooled -= it->first;
+        c->arena_pool.erase(it);
+        return EP_OK;
     }
-    if ((rc = upload_T(c, D->dev.spec_x, sx, prec))) return rc;
-    if ((rc = upload_T(c, D->dev.spec_y, sy, prec))) return rc;
+    // make room: pooled arenas go back to the device, largest first
+    while (!c->arena_pool.empty() && c->used + c->pooled + bytes > c->budget) {
+        auto last = std::prev(c->arena_pool.end());
+        hipFree(last->second);
+        c->pooled -= last->first;
+        c->arena_pool.erase(last);
+    }
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) return fail(c, EP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    *got = bytes;
+    return EP_OK;
+}
+
+void arena_release(Ctx* c, void* p, size_t bytes) {
+    // keep up to a quarter of the budget (at most 16 GiB) for reuse
+    const uint64_t cap = std::min<uint64_t>(c->budget / 4, 16ull << 30);
+    if (c->pooled + bytes <= cap) {
+        c->arena_pool.emplace(bytes, p);
+        c->pooled += bytes;
+    } else {
+        hipFree(p);
+    }
+}
+
+void arena_pool_clear(Ctx* c) {
+    for (auto& kv : c->arena_pool) hipFree(kv.second);
+    c->arena_pool.clear();
+    c->pooled = 0;
+}
+
+int commit_design(Ctx* c, Prepared& P, uint32_t* id_out) {
+    HIPCHK(c, hipSetDevice(c->device));
+    Design& D = *P.D;
+    if (c->used + P.bytes > c->budget) return fail(c, EP_ERR_OOM, "design does not fit in the HBM budget");
+    if (c->free_states.empty()) return fail(c, EP_ERR_OOM, "state pool exhausted");
+    size_t got = 0;
+    int rc = arena_alloc(c, P.bytes, &D.arena, &got);
+    if (rc) return rc;
+    D.bytes = got;
+    plan_arena(D, (char*)D.arena);
+    const size_t N = (size_t)D.dev.n_mov;
+    const NetLayout& L = P.L;
+    hipError_t e = hipMemsetAsync(D.arena, 0, P.bytes, c->stream);
+    if (e != hipSuccess) rc = fail(c, EP_ERR_DEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    for (int ax = 0; ax < 2 && !rc; ax++) {
+        rc = copy_in(c, D.dev.x0[ax], P.x0[ax].data(), P.x0[ax].size());
+        if (!rc) rc = copy_in(c, D.dev.fixed[ax], P.fixed[ax].data(), P.fixed[ax].size());
+        if (!rc) rc = copy_in(c, ax == 0 ? D.dev.spec_x : D.dev.spec_y, P.spec[ax].data(), P.spec[ax].size());
+        if (!rc) rc = copy_in(c, D.dev.perm[ax], P.ident.data(), N * 4);
+        if (!rc) rc = copy_in(c, D.dev.sorted_hs[ax], P.hs.data(), N * 4);
+    }
+    if (!rc) rc = copy_in(c, D.dev.pinw, P.pinw.data(), P.pinw.size());
+    if (!rc) rc = copy_in(c, D.dev.hs, P.hs.data(), N * 4);
+    if (!rc) rc = copy_in(c, D.dev.slots, L.slots.data(), L.slots.size() * 4);
+    if (!rc) rc = copy_in(c, D.dev.hf_off, L.hf_off.data(), L.hf_off.size() * 4);
+    if (!rc) rc = copy_in(c, D.dev.hf_pins, L.hf_pins.data(), L.hf_pins.size() * 4);
+    if (!rc) rc = copy_in(c, D.dev.cs_off, L.cs_off.data(), L.cs_off.size() * 4);
+    if (!rc) rc = copy_in(c, D.dev.cs_idx, L.cs_idx.data(), L.cs_idx.size() * 4);
+    if (rc) {
+        hipStreamSynchronize(c->stream);
+        arena_release(c, D.arena, D.bytes);
+        D.arena = nullptr;
+        return rc;
+    }
+    D.state_idx = c->free_states.back();
+    c->free_states.pop_back();
+    D.dev.st = c->state_pool + D.state_idx;
+    D.id = c->next_id++;
+    c->used += D.bytes;
+    *id_out = D.id;
+    c->designs[D.id] = std::move(P.D);
     return EP_OK;
 }
 
 int upload_locked(Ctx* c, const ep_design* d, int prec, int log_cap, uint32_t* id_out) {
-    HIPCHK(c, hipSetDevice(c->device));
-    NetLayout L;
-    build_layout(d, L);
-    std::unique_ptr<Design> D(new Design());
-    D->prec = prec;
-    D->log_cap = log_cap;
-    fill_desc_sizes(*D, d, L);
-    const size_t bytes = plan_arena(*D, nullptr);
-    if (c->used + bytes > c->budget)
-        return fail(c, EP_ERR_OOM, "design does not fit in the HBM budget");
-    if (c->free_states.empty()) return fail(c, EP_ERR_OOM, "state pool exhausted");
-    hipError_t e = hipMalloc(&D->arena, bytes);
-    if (e != hipSuccess) return fail(c, EP_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    D->bytes = bytes;
-    plan_arena(*D, (char*)D->arena);
-    e = hipMemset(D->arena, 0, bytes);
-    int rc = e == hipSuccess ? fill_arena(c, D.get(), d, L)
-                             : fail(c, EP_ERR_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
-    if (rc) {
-        hipFree(D->arena);
-        return rc;
-    }
-    D->state_idx = c->free_states.back();
-    c->free_states.pop_back();
-    D->dev.st = c->state_pool + D->state_idx;
-    D->id = c->next_id++;
-    c->used += bytes;
-    *id_out = D->id;
-    c->designs[D->id] = std::move(D);
-    return EP_OK;
+    Prepared P;
+    int rc = prepare_design(c, d, prec, log_cap, P);
+    if (!rc) rc = commit_design(c, P, id_out);
+    if (!rc) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return rc;
 }
 
 }  // namespace ep

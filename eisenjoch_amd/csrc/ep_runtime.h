@@ -93,6 +93,8 @@ struct Ctx : ep_ctx {
     uint32_t next_id = 1;
     double t0 = 0.0;
     std::unique_ptr<Batch> cache;    // last built batch (graph kept for replays)
+    std::multimap<size_t, void*> arena_pool;  // freed arenas by size, reusable
+    uint64_t pooled = 0;                       // bytes held by arena_pool
     ep_config cache_cfg{};
     bool cache_cfg_valid = false;
     ep_config eval_cfg{};            // configuration of the ep_eval_* calls
@@ -104,7 +106,43 @@ struct Ctx : ep_ctx {
 int fail(Ctx* c, int code, const std::string& msg);
 double now_ms();
 int get_transform(Ctx* c, int n, int mode, Transform** out);
+// Net slots of a design (nets of <= 64 pins in groups of 2..64, padded to
+// whole waves), its high-fanout nets and the cell -> slot CSR.
+struct NetLayout {
+    std::vector<uint32_t> slots;
+    int n_buckets = kMaxBuckets;
+    int bucket_wave_start[kMaxBuckets + 1] = {0};
+    int bucket_group[kMaxBuckets] = {0};
+    std::vector<uint32_t> hf_off, hf_pins;
+    std::vector<uint32_t> cs_off, cs_idx;
+};
+
+// Everything an upload computes on the host (layout, sizes, the initial
+// contents of the arena in the arithmetic type), so it can run on the
+// submitting thread while the device keeps working.  The vectors must stay
+// alive until the stream that copies them has been synchronised.
+struct Prepared {
+    std::unique_ptr<Design> D;  // sizes filled in, no arena yet
+    size_t bytes = 0;           // arena size
+    NetLayout L;
+    std::vector<char> x0[2], fixed[2], pinw, spec[2];  // arithmetic type T
+    std::vector<float> hs;
+    std::vector<uint32_t> ident;
+};
+
+// Validates d and fills P (host only; c receives the error, may be null).
+int prepare_design(Ctx* c, const ep_design* d, int prec, int log_cap, Prepared& P);
+// Allocates the arena (from the pool when it can), queues the zeroing and
+// the copies of P on the context stream and registers the design.  Caller
+// holds c->mu and keeps P alive until the stream is synchronised.
+int commit_design(Ctx* c, Prepared& P, uint32_t* id_out);
 int upload_locked(Ctx* c, const ep_design* d, int prec, int log_cap, uint32_t* id_out);
+// Device arenas are pooled: a freed arena is kept (up to a cap) for the next
+// design of about its size, so serving does not pay hipMalloc/hipFree (and
+// the device synchronisation of hipFree) per request.
+int arena_alloc(Ctx* c, size_t bytes, void** out, size_t* got);
+void arena_release(Ctx* c, void* p, size_t bytes);
+void arena_pool_clear(Ctx* c);
 int free_design_locked(Ctx* c, uint32_t id);
 void release_batch_cache(Ctx* c);
 

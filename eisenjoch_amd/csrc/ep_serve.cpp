@@ -31,10 +31,8 @@ enum ReqState { kQueued = 0, kRunning = 1, kDone = 2 };
 
 struct ServeReq {
     uint64_t ticket = 0;
-    std::vector<double> x0, y0, fx, fy, pinw;
-    std::vector<float> area;
-    std::vector<uint32_t> off, pins;
-    ep_design d{};
+    std::unique_ptr<Prepared> prep;  // built on the submitting thread; kept until done
+    int n_mov = 0;
     ep_config cfg{};
     int state = kQueued;
     int status = EP_PENDING;
@@ -143,7 +141,7 @@ static void serve_loop(Ctx* c) {
         std::lock_guard<std::recursive_mutex> g(c->mu);
         for (ServeReq* r : admit) {
             uint32_t id = 0;
-            int rc = upload_locked(c, &r->d, EP_PREC_FAST, std::max(1, r->cfg.max_iters), &id);
+            int rc = commit_design(c, *r->prep, &id);
             std::lock_guard<std::mutex> lk(S.m);
             r->res.t_admit_ms = now_ms() - c->t0;
             if (rc) {
@@ -196,6 +194,7 @@ static void serve_loop(Ctx* c) {
             if (cancel) res.status = EP_ERR_CANCELLED;
             res.t_done_ms = now_ms() - c->t0;
             free_design_locked(c, r->D->id);
+            r->prep.reset();  // its copies completed at the last synchronisation
             std::lock_guard<std::mutex> lk(S.m);
             r->D = nullptr;
             r->xo.swap(xo);
@@ -265,28 +264,14 @@ int ep_serve_submit(ep_ctx* h, const ep_design* d, const ep_config* cfg, uint64_
     int rc = validate_config(c, cfg, 1 << 20);
     if (rc) return rc;
     std::unique_ptr<ServeReq> r(new ServeReq());
-    const size_t N = (size_t)std::max(0, d->n_mov), F = (size_t)std::max(0, d->n_fixed);
-    r->x0.assign(d->x0, d->x0 + N);
-    r->y0.assign(d->y0, d->y0 + N);
-    if (F) {
-        r->fx.assign(d->fixed_x, d->fixed_x + F);
-        r->fy.assign(d->fixed_y, d->fixed_y + F);
-    }
-    if (d->area) r->area.assign(d->area, d->area + N);
-    if (d->pinw) r->pinw.assign(d->pinw, d->pinw + N);
-    if (d->net_off) r->off.assign(d->net_off, d->net_off + d->n_nets + 1);
-    if (d->n_pins) r->pins.assign(d->pin_cell, d->pin_cell + d->n_pins);
-    r->d = *d;
-    r->d.x0 = r->x0.data();
-    r->d.y0 = r->y0.data();
-    r->d.fixed_x = F ? r->fx.data() : nullptr;
-    r->d.fixed_y = F ? r->fy.data() : nullptr;
-    r->d.area = d->area ? r->area.data() : nullptr;
-    r->d.pinw = d->pinw ? r->pinw.data() : nullptr;
-    r->d.net_off = d->net_off ? r->off.data() : nullptr;
-    r->d.pin_cell = d->n_pins ? r->pins.data() : nullptr;
+    // layout, sizes and initial contents on the caller's thread, so the
+    // serving thread only allocates and queues copies
+    r->prep.reset(new Prepared());
+    rc = prepare_design(c, d, EP_PREC_FAST, std::max(1, cfg->max_iters), *r->prep);
+    if (rc) return rc;
+    r->n_mov = d->n_mov;
     r->cfg = *cfg;
-    r->bytes = ep_design_bytes(&r->d, EP_PREC_FAST);
+    r->bytes = r->prep->bytes;
     if (r->bytes > c->budget) return fail(c, EP_ERR_ADMISSION_REFUSED, "design exceeds the HBM budget");
     Server& S = *c->server;
     std::lock_guard<std::mutex> lk(S.m);
@@ -320,7 +305,7 @@ int ep_serve_positions(ep_ctx* h, uint64_t ticket, double* x, double* y) {
     auto it = S.all.find(ticket);
     if (it == S.all.end()) return fail(c, EP_ERR_NOT_FOUND, "unknown ticket");
     if (it->second->state != kDone) return EP_PENDING;
-    if (it->second->xo.empty() && it->second->d.n_mov > 0)
+    if (it->second->xo.empty() && it->second->n_mov > 0)
         return fail(c, it->second->status ? it->second->status : EP_ERR_NOT_FOUND, "no positions");
     std::copy(it->second->xo.begin(), it->second->xo.end(), x);
     std::copy(it->second->yo.begin(), it->second->yo.end(), y);
