@@ -36,15 +36,17 @@ __device__ inline int find_job(const GemmJob* jobs, int nj, int blk) {
 
 // Store one output element; kEpiPoisson applies the spectral scaling and
 // the Poisson solve of density.rs:113-153 to coefficient (ky = row, kx = col).
+// po: element offset of the output plane (K slice) being written.
 template <typename T>
-__device__ inline void epi_store(const GemmJob& J, const DesignDev& D, int row, int col, double v) {
+__device__ inline void epi_store(const GemmJob& J, const DesignDev& D, int row, int col, double v,
+                                 size_t po = 0) {
     if (row >= J.M || col >= J.N) return;
     if (J.epi == kEpiStore) {
-        ((T*)J.C)[(size_t)row * J.N + col] = (T)v;
+        ((T*)J.C)[po + (size_t)row * J.N + col] = (T)v;
     } else if (J.epi == kEpiStoreT) {
-        ((T*)J.C)[(size_t)col * J.M + row] = (T)v;
+        ((T*)J.C)[po + (size_t)col * J.M + row] = (T)v;
     } else if (J.epi == kEpiStore2) {
-        ((T*)J.C)[2 * ((size_t)row * J.N + col)] = (T)v;
+        ((T*)J.C)[po + 2 * ((size_t)row * J.N + col)] = (T)v;
     } else {
         const T* sx = (const T*)D.spec_x;  // [kx] -> (w, 2(1 - cos w))
         const T* sy = (const T*)D.spec_y;
@@ -61,8 +63,8 @@ __device__ inline void epi_store(const GemmJob& J, const DesignDev& D, int row, 
             ex = phi * wx;
             ey = phi * wy;
         }
-        ((T*)J.C)[(size_t)row * J.N + col] = (T)ex;
-        ((T*)J.C2)[(size_t)row * J.N + col] = (T)ey;
+        ((T*)J.C)[po + (size_t)row * J.N + col] = (T)ex;
+        ((T*)J.C2)[po + (size_t)row * J.N + col] = (T)ey;
     }
 }
 
@@ -134,8 +136,11 @@ struct Frag {
     bf16x8 hi[TK / 32], lo[TK / 32];
 };
 
+// A data operand written by a split pass is the sum of its `parts` planes
+// (pstride elements apart), added in plane order.
 template <int TK>
-__device__ inline Frag<TK> load_frag(int kind, const void* p, const void* plo, int r0, int k0, int K) {
+__device__ inline Frag<TK> load_frag(int kind, const void* p, const void* plo, int r0, int k0, int K,
+                                     int parts, int64_t pstride) {
     const int tid = threadIdx.x;
     const int row = tid >> 2, kc = (tid & 3) * 8;
     Frag<TK> f;
@@ -144,7 +149,13 @@ __device__ inline Frag<TK> load_frag(int kind, const void* p, const void* plo, i
         const size_t off = (size_t)(r0 + row) * K + k0 + kc + 32 * c;
         if (kind == 0) {
             const float4* src = (const float4*)((const float*)p + off);
-            const float4 a = src[0], b = src[1];
+            float4 a = src[0], b = src[1];
+            for (int q = 1; q < parts; q++) {
+                const float4* sq = (const float4*)((const float*)p + off + q * pstride);
+                const float4 a2 = sq[0], b2 = sq[1];
+                a.x += a2.x; a.y += a2.y; a.z += a2.z; a.w += a2.w;
+                b.x += b2.x; b.y += b2.y; b.z += b2.z; b.w += b2.w;
+            }
             const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
             split8(v, f.hi[c], f.lo[c]);
         } else {
@@ -181,7 +192,11 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, 
     const GemmJob& J = jobs[j];
     const DesignDev& D = dd[J.design];
     if (D.st->done) return;
-    const int t = tb - J.tile_start;
+    // block -> (K slice, tile): consecutive blocks take different tiles
+    const int tiles = ((J.M + 63) / 64) * J.tiles_n;
+    const int tt = tb - J.tile_start;
+    const int ks = tt / tiles, t = tt - ks * tiles;
+    const int kper = J.K / J.ksplit, kbeg = ks * kper;
     const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int wm = wv >> 1, wn = wv & 1;
@@ -191,17 +206,17 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, 
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = 0.0f;
     const int r = lane & 31, kh = (lane >> 5) * 8;
-    const int stages = J.K / TK;
+    const int stages = kper / TK;
     // stage 0 goes to LDS; stage 1 waits in registers (fa, fb); each stage
     // then issues the loads of the stage two ahead (na, nb), so two stages'
     // loads are in flight while one is multiplied
-    Frag<TK> fa = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, 0, J.K);
-    Frag<TK> fb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, 0, J.K);
+    Frag<TK> fa = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, kbeg, J.K, J.a_parts, J.a_pstride);
+    Frag<TK> fb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, kbeg, J.K, J.b_parts, J.b_pstride);
     store_frag<TK>(sA[0], fa);
     store_frag<TK>(sB[0], fb);
     if (stages > 1) {
-        fa = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, TK, J.K);
-        fb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, TK, J.K);
+        fa = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, kbeg + TK, J.K, J.a_parts, J.a_pstride);
+        fb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, kbeg + TK, J.K, J.b_parts, J.b_pstride);
     }
     __syncthreads();
     for (int st = 0; st < stages; st++) {
@@ -209,8 +224,10 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, 
         const bool more = st + 1 < stages;
         Frag<TK> na, nb;
         if (st + 2 < stages) {
-            na = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, (st + 2) * TK, J.K);
-            nb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, (st + 2) * TK, J.K);
+            na = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, kbeg + (st + 2) * TK, J.K, J.a_parts,
+                               J.a_pstride);
+            nb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, kbeg + (st + 2) * TK, J.K, J.b_parts,
+                               J.b_pstride);
         }
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 16) {
@@ -233,12 +250,30 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, 
         fb = nb;
     }
     // accumulator layout: element q of lane l is (row 8*(q/4) + 4*(l/32) + q%4, col l%32)
+    const size_t po = (size_t)ks * (size_t)J.c_pstride;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int row = m0 + wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
         const int col = n0 + wn * 32 + (lane & 31);
-        epi_store<float>(J, D, row, col, (double)acc[q]);
+        epi_store<float>(J, D, row, col, (double)acc[q], po);
     }
+}
+
+// Plane 0 of a split pass's output becomes the sum of its planes, added in
+// plane order (one block row per split design).
+__global__ __launch_bounds__(256) void k_plane_sum(const PlaneSum* sums) {
+    const PlaneSum S = sums[blockIdx.y];
+    if ((int)blockIdx.x >= S.blocks) return;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= S.n) return;
+    float v = S.p[i];
+    for (int q = 1; q < S.planes; q++) v += S.p[i + q * S.n];
+    S.p[i] = v;
+}
+
+void launch_plane_sum(const PlaneSum* sums, int n_sums, int max_blocks, hipStream_t s) {
+    if (n_sums <= 0 || max_blocks <= 0) return;
+    hipLaunchKernelGGL(k_plane_sum, dim3(max_blocks, n_sums), dim3(256), 0, s, sums);
 }
 
 // Split-K variant: the four waves of a block each compute the whole 64 x 64
@@ -325,6 +360,8 @@ void launch_gemm_pass(const GemmJob* jobs, int n_jobs, int total_tiles, const De
         hipLaunchKernelGGL(k_gemm_mfma<32>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
     else if (engine == 3)
         hipLaunchKernelGGL(k_gemm_mfma<64>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+    else if (engine == 4)  // K-sliced jobs
+        hipLaunchKernelGGL(k_gemm_mfma<32>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
     else if (engine == 2)
         hipLaunchKernelGGL(k_gemm_mfma_sk, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
     else

@@ -22,6 +22,8 @@ constexpr int kBinParts = 2;         // partial sums per bin block
 constexpr double kFixScale = 4294967296.0;  // 2^32 fixed point for deterministic sums
 constexpr int kScatCells = 1024;     // cells per scatter block
 constexpr int kScatBins = 6144;      // LDS accumulation window of a scatter block (bins)
+constexpr int kSplitMinK = 512;   // K from which transform products are split
+constexpr int kSplitSlice = 128;  // K per slice of a split product
 
 // Field modes (the transform constants differ, nothing else).
 enum FieldMode : int32_t { kFieldReference = 0, kFieldIdxst = 1 };
@@ -104,10 +106,14 @@ struct DesignDev {
     // bins
     void* dacc;                // FAST: u64 fixed point; FP64: double [By*Bx]
     void* dens;                // T [By*Bx]
-    void* tT;                  // F1 output, transposed [Bx*By]
-    void* ehat[2];             // F2 output [By*Bx]
-    void* uT[2];               // I1 output, transposed [Bx*By]
-    void* field;               // I2 output, T [By*Bx][2]: Ex and Ey interleaved
+    // transform intermediates; a pass whose products are split along K
+    // writes ks planes of partial sums (ks_x for K = Bx, ks_y for K = By)
+    void* tT;                  // F1 output, transposed [ks_x][Bx*By]
+    void* ehat[2];             // F2 output [ks_y][By*Bx]
+    void* uT[2];               // I1 output, transposed [ks_x][Bx*By]
+    void* field;               // I2 output, T [ks_y][By*Bx][2]: Ex and Ey interleaved; plane 0
+                               // holds the sum after the transforms
+    int32_t ks_x, ks_y;        // K slices of the products with K = Bx / K = By (1: unsplit)
     void* spec_x;              // T[2*Bx]: (w, 2(1-cos w)) per kx
     void* spec_y;              // T[2*By]
     // reductions
@@ -133,8 +139,20 @@ struct GemmJob {
     int32_t epi;               // 0: store, 1: store transposed, 2: Poisson epilogue
     int32_t design;            // index into the descriptor array
     int32_t tiles_n;           // tiles along N
-    int32_t tile_start;        // first global tile of this job
+    int32_t tile_start;        // first global block of this job
+    int32_t ksplit;            // K slices (one block per tile and slice), MFMA engines 0/3
+    int32_t a_parts, b_parts;  // partial planes summed, in order, when loading a data operand
     int32_t pad;
+    int64_t c_pstride;         // elements between the output's partial planes (C and C2)
+    int64_t a_pstride, b_pstride;  // elements between a data operand's planes
+};
+
+// Sum of the partial planes of a split pass, in place into plane 0.
+struct PlaneSum {
+    float* p;
+    int64_t n;        // elements per plane
+    int32_t planes;
+    int32_t blocks;   // blocks of 256 threads covering n
 };
 
 // kEpiStore2: row-major into every other element (one component of a pair)
@@ -174,6 +192,7 @@ void launch_wa_reduce(const BatchDev& b, int prec, int total_cell_blocks, double
 
 // ---- transforms (ep_gemm.hip) ----
 // Runs one pass (all jobs) of the spectral solve; engine 0 = MFMA, 1 = FMA.
+void launch_plane_sum(const PlaneSum* sums, int n_sums, int max_blocks, hipStream_t s);
 void launch_gemm_pass(const GemmJob* jobs_dev, int n_jobs, int total_tiles,
                       const DesignDev* dd, int prec, int engine, hipStream_t s);
 

@@ -82,6 +82,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     pc[0] = pb[0] = pw[0] = ph[0] = ps[0] = pt[0] = 0;
     // jobs: [pass][engine]
     std::vector<GemmJob> jobs[4][kEngines];
+    std::vector<PlaneSum> sums;  // split final passes: planes summed after the transforms
     for (int i = 0; i < nd; i++) {
         Design& D = *ds[i];
         dv[i] = D.dev;
@@ -113,6 +114,8 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             memset(&J, 0, sizeof(J));
             J.M = M; J.N = N; J.K = K; J.epi = epi; J.C = C; J.C2 = C2; J.design = i;
             J.tiles_n = (N + 63) / 64;
+            J.ksplit = 1;
+            J.a_parts = J.b_parts = 1;
             return J;
         };
         GemmJob f1 = mk(By, Bx, Bx, kEpiStoreT, D.dev.tT, nullptr);  // d . C2^T   (rows)
@@ -128,6 +131,28 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         GemmJob i2b = mk(By, Bx, By, kEpiStore2, (char*)D.dev.field + tsz, nullptr);  // S3 . U (idst_cols)
         cst(i2b, true, ty, 2); dat(i2b, false, D.dev.uT[1]);
         const int eng = engine_for(c, b.prec, Bx, By);
+        if (eng == 4) {
+            // split the long products along K (planes of partial sums); the
+            // next pass sums a data operand's planes as it loads it
+            const int64_t pl = (int64_t)Bx * By;
+            const int sx = D.dev.ks_x, sy = D.dev.ks_y;
+            f1.ksplit = sx; f1.c_pstride = pl;
+            f2.ksplit = sy; f2.c_pstride = pl; f2.b_parts = sx; f2.b_pstride = pl;
+            for (GemmJob* J : {&i1a, &i1b}) {
+                J->ksplit = sx; J->c_pstride = pl; J->a_parts = sy; J->a_pstride = pl;
+            }
+            for (GemmJob* J : {&i2a, &i2b}) {
+                J->ksplit = sy; J->c_pstride = 2 * pl; J->b_parts = sx; J->b_pstride = pl;
+            }
+            if (sy > 1) {
+                PlaneSum ps;
+                ps.p = (float*)D.dev.field;
+                ps.n = 2 * pl;
+                ps.planes = sy;
+                ps.blocks = (int)((ps.n + 255) / 256);
+                sums.push_back(ps);
+            }
+        }
         jobs[0][eng].push_back(f1);
         jobs[1][eng].push_back(f2);
         jobs[2][eng].push_back(i1a);
@@ -148,12 +173,13 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             int t0 = 0;
             for (auto& J : jobs[p][e]) {
                 J.tile_start = t0;
-                t0 += ((J.M + 63) / 64) * J.tiles_n;
+                t0 += ((J.M + 63) / 64) * J.tiles_n * J.ksplit;
             }
             b.tiles2[p][e] = t0;
             b.njobs2[p][e] = (int)jobs[p][e].size();
             bytes += align_up(sizeof(GemmJob) * std::max<size_t>(1, jobs[p][e].size()));
         }
+    bytes += align_up(sizeof(PlaneSum) * std::max<size_t>(1, sums.size()));
     HIPCHK(c, hipMalloc(&b.mem, bytes));
     std::vector<char> img(bytes, 0);
     size_t off = 0;
@@ -181,6 +207,15 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             if (jobs[p][e].empty()) b.jobs2[p][e] = (GemmJob*)put(&dummy, sizeof(GemmJob));
             else b.jobs2[p][e] = (GemmJob*)put(jobs[p][e].data(), sizeof(GemmJob) * jobs[p][e].size());
         }
+    {
+        PlaneSum none;
+        memset(&none, 0, sizeof(none));
+        b.sums = (PlaneSum*)(sums.empty() ? put(&none, sizeof(none))
+                                          : put(sums.data(), sizeof(PlaneSum) * sums.size()));
+        b.n_sums = (int)sums.size();
+        b.sum_blocks = 0;
+        for (auto& ps : sums) b.sum_blocks = std::max(b.sum_blocks, ps.blocks);
+    }
     HIPCHK(c, hipMemcpy(b.mem, img.data(), bytes, hipMemcpyHostToDevice));
     b.cfgs.assign(nd, ep_config{});
     for (int i = 0; i < nd; i++) b.cfgs[i] = *cfgs[i];
@@ -251,6 +286,7 @@ static void enqueue_gemm(Ctx* c, Batch& b, int p) {
     for (int e = 0; e < kEngines; e++)
         if (b.njobs2[p][e] > 0)
             launch_gemm_pass(b.jobs2[p][e], b.njobs2[p][e], b.tiles2[p][e], b.d_dev, b.prec, e, c->stream);
+    if (p == 3) launch_plane_sum(b.sums, b.n_sums, b.sum_blocks, c->stream);  // field = sum of planes
 }
 
 // Phases of one iteration, in launch order (names used by the profiler).
@@ -639,7 +675,9 @@ extern "C" int ep_autotune(ep_ctx* h, const uint32_t* ids, uint32_t n, uint32_t 
     for (auto& sh : todo) {
         float best = 1e30f;
         int best_e = c->tune[sh];
-        for (int e : {0, 3, 2, 1}) {
+        const bool splittable = sh.first >= kSplitMinK || sh.second >= kSplitMinK;
+        for (int e : {0, 3, 4, 2, 1}) {
+            if (e == 4 && !splittable) continue;  // same launches as engine 0
             c->tune[sh] = e;
             float ms = 0.f;
             int rc = time_transforms(c, ds, reps, &ms);
@@ -920,10 +958,7 @@ int ep_eval_field(ep_ctx* h, uint32_t id, const double* dens, double* ex, double
     if (rc) return rc;
     const size_t NB = (size_t)D->dev.Bx * D->dev.By;
     if ((rc = put_T(c, D->dev.dens, dens, NB, D->prec))) return rc;
-    for (int p = 0; p < 4; p++)
-        for (int e = 0; e < kEngines; e++)
-            if (b->njobs2[p][e] > 0)
-                launch_gemm_pass(b->jobs2[p][e], b->njobs2[p][e], b->tiles2[p][e], b->d_dev, b->prec, e, c->stream);
+    for (int p = 0; p < 4; p++) enqueue_gemm(c, *b, p);
     HIPCHK(c, hipGetLastError());
     std::vector<double> xy(2 * NB);
     if ((rc = get_T(c, xy.data(), D->dev.field, 2 * NB, D->prec))) return rc;

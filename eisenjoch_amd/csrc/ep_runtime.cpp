@@ -274,10 +274,10 @@ static size_t plan_arena(Design& D, char* base) {
     G.hf_pins = (uint32_t*)P(D.hf_pin_count * 4);
     G.dacc = P(B * 8);
     G.dens = P(B * T);
-    G.tT = P(B * T);
-    for (int ax = 0; ax < 2; ax++) G.ehat[ax] = P(B * T);
-    for (int ax = 0; ax < 2; ax++) G.uT[ax] = P(B * T);
-    G.field = P(B * 2 * T);
+    G.tT = P(B * T * G.ks_x);
+    for (int ax = 0; ax < 2; ax++) G.ehat[ax] = P(B * T * G.ks_y);
+    for (int ax = 0; ax < 2; ax++) G.uT[ax] = P(B * T * G.ks_x);
+    G.field = P(B * 2 * T * G.ks_y);
     G.spec_x = P((size_t)2 * G.Bx * T);
     G.spec_y = P((size_t)2 * G.By * T);
     G.part_cell = (double*)P((size_t)G.n_cell_blocks * kCellParts * 8);
@@ -298,6 +298,11 @@ static void fill_desc_sizes(Design& D, const ep_design* d, const NetLayout& L) {
     G.Bx = d->Bx > 0 ? d->Bx : d->W;
     G.By = d->By > 0 ? d->By : d->H;
     G.prec = D.prec;
+    // K slices of the long transform products (FAST MFMA shapes, K >= 512):
+    // slices of 128 keep each block's chain of K stages short
+    const bool mfma_shape = D.prec == EP_PREC_FAST && G.Bx % 64 == 0 && G.By % 64 == 0;
+    G.ks_x = mfma_shape && G.Bx >= kSplitMinK ? G.Bx / kSplitSlice : 1;
+    G.ks_y = mfma_shape && G.By >= kSplitMinK ? G.By / kSplitSlice : 1;
     G.n_cell_blocks = (G.n_mov + kBlock - 1) / kBlock;
     G.n_scat_blocks = (G.n_mov + kScatCells - 1) / kScatCells;
     G.n_bin_blocks = (G.Bx * G.By + kBlock - 1) / kBlock;

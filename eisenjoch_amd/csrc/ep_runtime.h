@@ -19,8 +19,10 @@ namespace ep {
 
 constexpr int kStatePool = 8192;      // resident designs per context
 // Transform engines: 0 MFMA with LDS staging in K stages of 32, 1 FP32/FP64
-// FMA, 2 MFMA split-K, 3 MFMA with LDS staging in K stages of 64.
-constexpr int kEngines = 4;
+// FMA, 2 MFMA split-K inside a block, 3 MFMA with LDS staging in K stages of
+// 64, 4 engine 0 with products of K >= 512 split across blocks into K slices
+// of 128 (partial planes summed by the next pass).
+constexpr int kEngines = 5;
 constexpr int kDefaultLogCap = 2048;  // iteration records per design
 
 // Constant transform matrices of one size and field mode: [0] C2, [1] C3,
@@ -60,6 +62,8 @@ struct Batch {
     int njobs2[4][kEngines] = {};
     int tiles2[4][kEngines] = {};
     GemmJob* jobs2[4][kEngines] = {};
+    PlaneSum* sums = nullptr;        // split final passes (device), summed after pass 3
+    int n_sums = 0, sum_blocks = 0;
     void* mem = nullptr;        // descriptors, prefixes, jobs
     DesignDev* d_dev = nullptr;
     const int32_t* dzero = nullptr;  // a device zero: prefix of a one-design view

@@ -200,7 +200,9 @@ int ep_profile_iteration(ep_ctx* ctx, const uint32_t* ids, uint32_t n, const ep_
 /* Autotuner (no counterpart in the reference): for each distinct bin shape
  * among the FAST designs `ids`, largest first, times every eligible
  * transform engine (engine 0: MFMA with LDS staging in K stages of 32;
- * 1: FP32 FMA; 2: MFMA split-K; 3: MFMA with K stages of 64) on the four
+ * 1: FP32 FMA; 2: MFMA split-K inside a block; 3: MFMA with K stages of 64;
+ * 4: engine 0 with products of K >= 512 split across blocks into K slices
+ * of 128) on the four
  * transform passes of the whole batch `ids`, `reps` times, with the other
  * shapes at their current choice, and keeps the fastest per shape for every
  * later batch.  With a cache path, shapes found in the file are reused and
@@ -209,8 +211,8 @@ int ep_autotune(ep_ctx* ctx, const uint32_t* ids, uint32_t n, uint32_t reps, con
 /* The tuning table as (Bx, By, engine) triples; *n receives the entry count. */
 int ep_tune_table(ep_ctx* ctx, int32_t* out, uint32_t cap, uint32_t* n);
 /* Pins the transform engine of one bin shape in the tuning table (engine < 0
- * removes the entry; the MFMA engines 0, 2, 3 need bins that are multiples of
- * 64).  No counterpart in the reference. */
+ * removes the entry; the MFMA engines 0, 2, 3, 4 need bins that are multiples
+ * of 64).  No counterpart in the reference. */
 int ep_tune_set(ep_ctx* ctx, int32_t bx, int32_t by, int32_t engine);
 
 /* Copies `bytes` device-to-device once with the runtime's plain copy kernel

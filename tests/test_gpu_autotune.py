@@ -1,7 +1,7 @@
 """Transform engines and the autotuner.
 
-Every engine computes the same spectral field: on a random density each one
-matches the FP64 oracle's field to 1e-4 of its largest value (the BF16
+Every engine computes the same spectral field (split along K or not): on a
+random density each one matches the FP64 oracle's field to 1e-4 of its largest value (the BF16
 hi+lo tolerance of the FAST path; the FMA engine is FP32).  The autotuner
 fills one table entry per bin shape with an engine it timed, reuses a cache
 file, and a batch placed with the tuned table still meets the quality bar
@@ -17,9 +17,11 @@ from conftest import log_case
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("engine", [0, 1, 2, 3])
-@pytest.mark.parametrize("B", [64, 128, 256])
+@pytest.mark.parametrize("engine", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("B", [64, 128, 256, 512])
 def test_every_engine_gives_the_field(device, engine, B):
+    # at 512 bins engine 4 splits each product into four K slices whose
+    # partial planes the next pass (and the final plane sum) add
     d = ep.generate(64, 64, B, n_fixed=0, Bx=B, By=B, seed=2)
     log_case(f"engine {engine} field B={B}")
     device.eval_config(ep.ElectroPlaceCfg(field_mode="idxst"))

@@ -96,3 +96,23 @@ def test_serving_admission_refused_over_budget():
             assert r.status == 0
         finally:
             srv.stop()
+
+
+def test_serving_rejects_invalid_designs_at_submit():
+    # submit validates like ep_design_upload: a pin naming a cell that does
+    # not exist never reaches the device
+    d = ep.sized_design(2000, 9)
+    bad_pins = d.pin_cell.copy()
+    bad_pins[5] = d.n_mov + d.n_fixed + 10
+    bad = ep.FlatDesign(W=d.W, H=d.H, x0=d.x0, y0=d.y0, fixed_x=d.fixed_x, fixed_y=d.fixed_y,
+                        net_off=d.net_off, pin_cell=bad_pins, area=d.area, Bx=d.Bx, By=d.By)
+    with ep.Device(0) as dev:
+        log_case("serving: invalid design at submit")
+        srv = ep.Server(dev, max_slots=2)
+        try:
+            with pytest.raises(ValueError, match="pin_cell out of range"):
+                srv.submit(bad, CFG)
+            t = srv.submit(d, CFG)  # the server keeps serving
+            assert wait(srv, [t])[t].status == 0
+        finally:
+            srv.stop()
