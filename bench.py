@@ -257,7 +257,7 @@ KERNEL_GROUPS = {
            "note": "second stream: overlaps the tile sort, scatter and transforms"},
     "dct (4 transform passes, BF16 hi+lo MFMA)": {
         "phases": ["dct_f1", "dct_f2", "dct_i1", "dct_i2"], "bound": "mfma", "work": "dct_flops",
-        "kernels": {"ep::k_gemm_mfma(": 4}},
+        "kernels": {"ep::k_gemm_mfma<": 4}},
 }
 
 
