@@ -81,3 +81,50 @@ def test_autotune_table_cache_and_quality(device, tmp_path):
             device.free(i)
         for d in ds:
             device.tune_set(d.Bx, d.By, -1)
+
+
+def test_kslice_engine_placement_matches_unsplit(device):
+    # a 512-bin design placed with its products split into K slices (engine
+    # 4) and unsplit (engine 0): both reach overflow <= 0.10 with HPWL within
+    # 0.1 % of each other (the engines differ only in FP32 summation order)
+    d = ep.sized_design(100000, 31)
+    assert d.Bx == 512
+    cfg = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
+    i = device.upload(d, "fast")
+    try:
+        out = {}
+        for e in (0, 4):
+            log_case(f"placement with engine {e} at 512 bins")
+            device.tune_set(512, 512, e)
+            out[e] = device.place(i, cfg, positions=False)
+        for r in out.values():
+            assert r.converged and r.overflow <= 0.10
+        assert abs(out[4].hpwl - out[0].hpwl) / out[0].hpwl < 1e-3
+    finally:
+        device.free(i)
+        device.tune_set(512, 512, -1)
+
+
+def test_reused_arena_gives_the_fresh_result(device):
+    # a freed arena goes to the pool and the next design of about its size
+    # reuses it: the result must equal the same design placed in a fresh
+    # context, bit for bit (the arena is zeroed and refilled on reuse)
+    a = ep.sized_design(30000, 41)
+    b = ep.sized_design(29000, 42)
+    cfg = ep.ElectroPlaceCfg(max_iters=300, field_mode="idxst", log_hpwl=False)
+    log_case("arena reuse")
+    ia = device.upload(a, "fast")
+    device.place(ia, cfg, positions=False)
+    device.free(ia)
+    ib = device.upload(b, "fast")
+    try:
+        reused = device.place(ib, cfg)
+    finally:
+        device.free(ib)
+    with ep.Device(0) as fresh:
+        j = fresh.upload(b, "fast")
+        ref = fresh.place(j, cfg)
+        fresh.free(j)
+    assert reused.iters == ref.iters
+    np.testing.assert_array_equal(reused.x, ref.x)
+    np.testing.assert_array_equal(reused.y, ref.y)
