@@ -40,7 +40,7 @@ class EpDesign(C.Structure):
         ("W", C.c_int32), ("H", C.c_int32), ("Bx", C.c_int32), ("By", C.c_int32),
         ("reserved", C.c_int32), ("n_pins", C.c_int64),
         ("x0", _p), ("y0", _p), ("area", _p), ("fixed_x", _p), ("fixed_y", _p),
-        ("net_off", _p), ("pin_cell", _p), ("pinw", _p),
+        ("net_off", _p), ("pin_cell", _p), ("pinw", _p), ("net_weight", _p),
     ]
 
 

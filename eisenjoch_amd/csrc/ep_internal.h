@@ -85,6 +85,7 @@ struct DesignDev {
     void* pg;                  // per-slot WA gradient (gx, gy) in T [n_slots][2]
     uint32_t* cs_off;          // cell -> slot CSR [n_mov + 1]
     double* gwl;               // [n_mov][2] WA gradient per cell (its slots summed in FP64)
+    const void* slot_w;        // T [n_slots]: each slot's net weight, or null (all nets weigh 1)
     uint32_t* cs_idx;          // slots of each movable cell, ascending
     uint32_t* perm[2];         // cells in Morton order of their lower-left bin tile; the
                                // sort of iteration k reads perm[k&1] and writes perm[(k+1)&1]

@@ -121,6 +121,11 @@ __device__ inline void pin_pos(const DesignDev& D, uint32_t c, T& px, T& py) {
 template <typename T>
 __device__ inline void store_grad(const DesignDev& D, uint32_t slot, T gx, T gy) {
     typedef typename Vec2<T>::type V;
+    if (D.slot_w) {  // net weight, common.rs:246-261 (absent: 1)
+        const T w = ((const T*)D.slot_w)[slot];
+        gx = w * gx;
+        gy = w * gy;
+    }
     V v;
     v.x = gx;
     v.y = gy;

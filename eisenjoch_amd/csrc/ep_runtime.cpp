@@ -150,6 +150,7 @@ static void build_layout(const ep_design* d, NetLayout& L) {
         if (n > 64) {
             for (uint32_t k = 0; k < n; k++) L.hf_pins.push_back(d->pin_cell[b0 + k]);
             L.hf_off.push_back((uint32_t)L.hf_pins.size());
+            L.hf_net.push_back(e);
             continue;
         }
         int b = 0;
@@ -171,9 +172,15 @@ static void build_layout(const ep_design* d, NetLayout& L) {
         const size_t start = L.slots.size();
         for (auto& kv : nets[b]) {
             uint32_t b0 = d->net_off[kv.second], n = d->net_off[kv.second + 1] - b0;
-            for (uint32_t k = 0; k < G; k++) L.slots.push_back(k < n ? d->pin_cell[b0 + k] : kPad);
+            for (uint32_t k = 0; k < G; k++) {
+                L.slots.push_back(k < n ? d->pin_cell[b0 + k] : kPad);
+                L.slot_net.push_back(k < n ? kv.second : -1);
+            }
         }
-        while ((L.slots.size() - start) % kWave) L.slots.push_back(kPad);
+        while ((L.slots.size() - start) % kWave) {
+            L.slots.push_back(kPad);
+            L.slot_net.push_back(-1);
+        }
         waves += (int)((L.slots.size() - start) / kWave);
     }
     L.bucket_wave_start[kMaxBuckets] = waves;
@@ -225,6 +232,10 @@ static int validate_design(Ctx* c, const ep_design* d) {
         for (int i = 0; i < d->n_mov; i++)
             if (!(d->area[i] > 0.0f) || !std::isfinite(d->area[i]))
                 return fail(c, EP_ERR_INVALID_ARG, "area must be positive and finite");
+    if (d->net_weight)
+        for (int e = 0; e < d->n_nets; e++)
+            if (!std::isfinite(d->net_weight[e]))
+                return fail(c, EP_ERR_INVALID_ARG, "net weights must be finite");
     return EP_OK;
 }
 
@@ -268,6 +279,7 @@ static size_t plan_arena(Design& D, char* base) {
     G.tcur = (uint32_t*)P((size_t)G.n_keys * 4);
     G.pinw = P(N * T);
     G.hs = (float*)P(N * 4);
+    G.slot_w = D.weighted ? P((size_t)G.n_slots * T) : nullptr;
     for (int ax = 0; ax < 2; ax++) G.fixed[ax] = P(F * T);
     G.slots = (uint32_t*)P((size_t)G.n_wa_waves * kWave * 4);
     G.hf_off = (uint32_t*)P((size_t)(G.n_hf + 1) * 4);
@@ -342,6 +354,7 @@ static void fill_desc_sizes(Design& D, const ep_design* d, const NetLayout& L) {
         }
     }
     D.cs_count = L.cs_off.empty() ? 0 : L.cs_off.back();
+    D.weighted = d->net_weight != nullptr;
 }
 
 }  // namespace ep
@@ -511,6 +524,16 @@ int prepare_design(Ctx* c, const ep_design* d, int prec, int log_cap, Prepared& 
     P.hs.resize(N);
     for (size_t i = 0; i < N; i++)
         P.hs[i] = (float)(0.5 * std::sqrt((double)(d->area ? d->area[i] : 1.0f)));
+    if (d->net_weight) {  // common.rs:246-249, per slot of the WA layout
+        std::vector<double> sw((size_t)D.dev.n_slots, 0.0);
+        for (size_t s = 0; s < P.L.slot_net.size(); s++)
+            if (P.L.slot_net[s] >= 0) sw[s] = d->net_weight[P.L.slot_net[s]];
+        const size_t base = P.L.slots.size();
+        for (size_t h = 0; h + 1 < P.L.hf_off.size(); h++)
+            for (uint32_t p = P.L.hf_off[h]; p < P.L.hf_off[h + 1]; p++)
+                sw[base + p] = d->net_weight[P.L.hf_net[h]];
+        to_prec(P.slot_w, sw.data(), sw.size(), D.prec);
+    }
     // scatter order: any permutation of the cells is valid; start with the identity
     P.ident.resize(N);
     for (size_t i = 0; i < N; i++) P.ident[i] = (uint32_t)i;
@@ -601,6 +624,7 @@ int commit_design(Ctx* c, Prepared& P, uint32_t* id_out) {
     if (!rc) rc = copy_in(c, D.dev.hf_pins, L.hf_pins.data(), L.hf_pins.size() * 4);
     if (!rc) rc = copy_in(c, D.dev.cs_off, L.cs_off.data(), L.cs_off.size() * 4);
     if (!rc) rc = copy_in(c, D.dev.cs_idx, L.cs_idx.data(), L.cs_idx.size() * 4);
+    if (!rc && D.dev.slot_w) rc = copy_in(c, (void*)D.dev.slot_w, P.slot_w.data(), P.slot_w.size());
     if (rc) {
         hipStreamSynchronize(c->stream);
         arena_release(c, D.arena, D.bytes);

@@ -45,6 +45,7 @@ struct Design {
     size_t bytes = 0;
     size_t hf_pin_count = 0;
     size_t cs_count = 0;            // movable pins (cell -> slot CSR entries)
+    bool weighted = false;          // per-net weights given (slot_w allocated)
     DesignDev dev{};
 };
 
@@ -119,6 +120,8 @@ struct NetLayout {
     int bucket_group[kMaxBuckets] = {0};
     std::vector<uint32_t> hf_off, hf_pins;
     std::vector<uint32_t> cs_off, cs_idx;
+    std::vector<int32_t> slot_net;  // net of each slot of `slots` (-1: padding)
+    std::vector<int32_t> hf_net;    // net of each high-fanout net
 };
 
 // Everything an upload computes on the host (layout, sizes, the initial
@@ -130,6 +133,7 @@ struct Prepared {
     size_t bytes = 0;           // arena size
     NetLayout L;
     std::vector<char> x0[2], fixed[2], pinw, spec[2];  // arithmetic type T
+    std::vector<char> slot_w;   // T per slot: its net's weight (empty: all nets weigh 1)
     std::vector<float> hs;
     std::vector<uint32_t> ident;
 };

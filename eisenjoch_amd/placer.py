@@ -144,6 +144,8 @@ class FlatDesign:
     bel_type: Optional[np.ndarray] = None
     init_bel: Optional[np.ndarray] = None
     name: str = ""
+    # optional per-net weights of the WA gradient (common.rs:204-261)
+    net_weight: Optional[np.ndarray] = None
 
     def __post_init__(self):
         self.x0 = np.ascontiguousarray(self.x0, dtype=np.float64)
@@ -156,6 +158,8 @@ class FlatDesign:
             self.area = np.ascontiguousarray(self.area, dtype=np.float32)
         if self.pinw is not None:
             self.pinw = np.ascontiguousarray(self.pinw, dtype=np.float64)
+        if self.net_weight is not None:
+            self.net_weight = np.ascontiguousarray(self.net_weight, dtype=np.float64)
         if not self.Bx:
             self.Bx = self.W
         if not self.By:
@@ -187,6 +191,7 @@ class FlatDesign:
         d.fixed_x, d.fixed_y = L.ptr(self.fixed_x), L.ptr(self.fixed_y)
         d.net_off, d.pin_cell = L.ptr(self.net_off), L.ptr(self.pin_cell)
         d.pinw = L.ptr(self.pinw)
+        d.net_weight = L.ptr(self.net_weight)
         return d
 
     def nbytes(self, precision: str = "fast") -> int:

@@ -73,6 +73,10 @@ typedef struct ep_design {
     const uint32_t* net_off;   /* [n_nets + 1] */
     const uint32_t* pin_cell;  /* [n_pins] */
     const double* pinw;        /* [n_mov] or NULL */
+    const double* net_weight;  /* [n_nets] or NULL (every net weighs 1): the optional net
+                                  weights of add_wa_wirelength_gradient (common.rs:204-261),
+                                  e.g. a timing-driven caller's 1 + timing_weight * crit^2;
+                                  ElectroPlace passes none (mod.rs:112-115) */
 } ep_design;
 
 /* ElectroPlaceCfg (electro_place/config.rs:5-40) plus the constants of

@@ -129,11 +129,13 @@ void orc_wa_gradient(const orc_design* d, const double* x, const double* y,
         }
         orc_wa_axis_grad(px, (int)n, a, ggx);
         orc_wa_axis_grad(py, (int)n, a, ggy);
+        /* common.rs:246-249: the net's weight, 1 when none is given */
+        const double w = d->net_weight ? d->net_weight[e] : 1.0;
         for (uint32_t k = 0; k < n; k++) {
             uint32_t c = d->pin_cell[b + k];
-            if (c < (uint32_t)d->n_mov) {
-                gx[c] += 1.0 * ggx[k];
-                gy[c] += 1.0 * ggy[k];
+            if (c < (uint32_t)d->n_mov) { /* common.rs:258-261 */
+                gx[c] += w * ggx[k];
+                gy[c] += w * ggy[k];
             }
         }
     }

@@ -34,6 +34,7 @@ typedef struct orc_design {
     const uint32_t* net_off; /* n_nets + 1 */
     const uint32_t* pin_cell;/* n_pins */
     const double* pinw;      /* NULL => derived from the CSR degrees */
+    const double* net_weight;/* NULL => every net weighs 1 (ElectroPlace passes None, mod.rs:114) */
 } orc_design;
 
 typedef struct orc_cfg {

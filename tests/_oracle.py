@@ -17,7 +17,8 @@ class OrcDesign(C.Structure):
     _fields_ = [("n_mov", C.c_int32), ("n_fixed", C.c_int32), ("n_nets", C.c_int32),
                 ("W", C.c_int32), ("H", C.c_int32), ("Bx", C.c_int32), ("By", C.c_int32),
                 ("n_pins", C.c_int64), ("x0", _p), ("y0", _p), ("area", _p), ("fixed_x", _p),
-                ("fixed_y", _p), ("net_off", _p), ("pin_cell", _p), ("pinw", _p)]
+                ("fixed_y", _p), ("net_off", _p), ("pin_cell", _p), ("pinw", _p),
+                ("net_weight", _p)]
 
 
 class OrcCfg(C.Structure):
@@ -111,7 +112,8 @@ class Design:
         self.y0 = np.ascontiguousarray(fd.y0 if y is None else y, np.float64)
         self.c = OrcDesign(fd.n_mov, fd.n_fixed, fd.n_nets, fd.W, fd.H, fd.Bx, fd.By, fd.n_pins,
                            P(self.x0), P(self.y0), P(fd.area), P(fd.fixed_x), P(fd.fixed_y),
-                           P(fd.net_off), P(fd.pin_cell), P(fd.pinw))
+                           P(fd.net_off), P(fd.pin_cell), P(fd.pinw),
+                           P(getattr(fd, "net_weight", None)))
 
     @property
     def ref(self):

@@ -63,6 +63,28 @@ def test_wa_gradient_parity(device, designs, prec, name):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fast"])
+def test_wa_gradient_net_weights_parity(device, designs, prec):
+    # optional per-net weights (common.rs:246-261), high-fanout nets included
+    d = designs["small_hf"]
+    w = np.random.default_rng(5).uniform(0.0, 6.0, d.n_nets)
+    dw = ep.FlatDesign(W=d.W, H=d.H, x0=d.x0, y0=d.y0, fixed_x=d.fixed_x, fixed_y=d.fixed_y,
+                       net_off=d.net_off, pin_cell=d.pin_cell, area=d.area, Bx=d.Bx, By=d.By,
+                       net_weight=w)
+    log_case(f"wa_gradient weighted {prec}")
+    i = device.upload(dw, prec)
+    try:
+        x, y = rand_pos(d, 3)
+        gx, gy = device.wa_grad(i, x, y)
+        ox, oy = O.wa_gradient(dw, x, y, 0.5)
+        scale = max(np.abs(ox).max(), np.abs(oy).max())
+        tol = 1e-12 if prec == "fp64" else 2e-5  # relative to the largest component
+        assert np.abs(gx - ox).max() <= tol * scale
+        assert np.abs(gy - oy).max() <= tol * scale
+    finally:
+        device.free(i)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fast"])
 def test_bin_ranges_bit_exact(device, prec):
     # bins != tiles and areas 1/2/4: integer ranges equal the oracle's exactly
     d = ep.generate(5000, 5500, 150, n_fixed=32, Bx=128, By=64, area_mix=True, seed=4)

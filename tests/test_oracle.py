@@ -126,6 +126,35 @@ def test_wa_gradient_net_assembly_brute_force():
     np.testing.assert_allclose(w, [0.5, 0.5 + 0.2, 1 / 3 + 0.2, 1 / 3 + 0.2, 0.2], rtol=1e-15)
 
 
+def test_wa_gradient_net_weights():
+    # common.rs:246-261: each net's pin gradients are scaled by its weight
+    # (1 for a net without one).  One net weighted at a time reproduces that
+    # net's own contribution times its weight; weights of 1 equal no weights.
+    rng = np.random.default_rng(2)
+    x = rng.uniform(0, 7, 5); y = rng.uniform(0, 7, 5)
+    nets = [(0, 1), (2, 3, 5), (4,), (1, 2, 3, 4, 6)]
+    d = tiny_design(x, y, nets=nets, fixed=[(0.0, 7.0), (7.0, 0.0)])
+    gx0, gy0 = O.wa_gradient(d, x, y, 0.5)
+    d.net_weight = np.ones(len(nets))
+    gx1, gy1 = O.wa_gradient(d, x, y, 0.5)
+    np.testing.assert_array_equal(gx1, gx0)
+    np.testing.assert_array_equal(gy1, gy0)
+    allx = np.concatenate([x, [0.0, 7.0]]); ally = np.concatenate([y, [7.0, 0.0]])
+    for e, n in enumerate(nets):
+        w = np.zeros(len(nets)); w[e] = 2.75
+        d.net_weight = w
+        gx, gy = O.wa_gradient(d, x, y, 0.5)
+        ex = np.zeros(5); ey = np.zeros(5)
+        if len(n) >= 2:
+            ggx = wa_axis_grad(allx[list(n)], 0.5)
+            ggy = wa_axis_grad(ally[list(n)], 0.5)
+            for k, c in enumerate(n):
+                if c < 5:
+                    ex[c] += 2.75 * ggx[k]; ey[c] += 2.75 * ggy[k]
+        np.testing.assert_allclose(gx, ex, rtol=1e-14, atol=1e-15)
+        np.testing.assert_allclose(gy, ey, rtol=1e-14, atol=1e-15)
+
+
 # ---------------------------------------------------------- transforms ----
 
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 12, 64, 100])
