@@ -5,8 +5,9 @@ log-uniform sizes of 2,000-100,000 movable cells (seed 0; design i uses seed
 i), nets = 1.1 x cells, grid side = round(sqrt(cells / 0.7)), bins = nearest
 power of two to the side in [64, 512], cell areas 1/2/4.  One step places the
 whole batch from its initial positions until every design reaches overflow
-<= 0.10 (checked every 5 iterations after iteration 20, as the reference
-does), all designs advancing together in one hipGraph-replayed launch
+<= 0.10 (from iteration 20 on; checked every iteration by default, every 5
+with --stop reference as the reference's loop does), all designs advancing
+together in one hipGraph-replayed launch
 sequence per iteration; FP32 bulk arithmetic, BF16 hi+lo MFMA transforms,
 FP64 reductions.  The spectral field is the index-shifted (IDXST) one: the
 reference's own field never reaches overflow 0.10 on these designs.
@@ -59,10 +60,24 @@ def parse():
                    help="batch (default), or one design: config1 (1,500 cells, 64x64, the "
                         "reference's own size), config2 (200k cells, 512 bins) or config3 "
                         "(1M cells, high-fanout nets, 1,024 bins)")
+    p.add_argument("--stop", default="every", choices=["every", "reference"],
+                   help="overflow check every iteration (default: the first iteration that "
+                        "reaches overflow <= 0.10 ends a design) or every 5 iterations as the "
+                        "reference's loop, which also legalizes there, does")
     p.add_argument("--traffic-file", default=str(ROOT / "profiles" / "pmc_traffic_batched64.json"),
                    help="per-dispatch FETCH_SIZE/WRITE_SIZE of the batched iteration's kernels, "
                         "from rocprofv3 --pmc runs of --profile-only (reported as roofline.traffic)")
     return p.parse_args()
+
+
+def place_cfg(args):
+    """The placement settings of every workload: IDXST field, at most 500
+    iterations, stop at overflow <= 0.10 from iteration 20 on, checked every
+    iteration (default) or every 5 iterations as the reference's loop does
+    (--stop reference).  The plain FP64 placer uses the same settings."""
+    import eisenjoch_amd as ep
+    return ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False,
+                              stop_every_iter=args.stop == "every")
 
 
 def single(args):
@@ -72,7 +87,7 @@ def single(args):
     against the FP64 field on the same density."""
     import eisenjoch_amd as ep
     d = {"config1": ep.config1, "config2": ep.config2, "config3": ep.config3}[args.workload]()
-    cfg = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
+    cfg = place_cfg(args)
     dev = ep.Device(0)
     ids = {p: dev.upload(d, p) for p in ("fp64", "fast")}
     tune = dev.autotune([ids["fast"]], reps=5)  # engine for this bin shape, before timing
@@ -92,7 +107,8 @@ def single(args):
         "value": round(1.0 / t32, 3), "unit": "designs/s", "n_gpus": 1, "steps": max(1, args.steps),
         "higher_is_better": True,
         "config": {"workload": d.name, "cells": d.n_mov, "nets": d.n_nets, "pins": d.n_pins,
-                   "bins": [d.Bx, d.By], "tiles": [d.W, d.H], "transform_engines": tune},
+                   "bins": [d.Bx, d.By], "tiles": [d.W, d.H], "transform_engines": tune,
+                   "stop_check": "every iteration" if args.stop == "every" else "every 5 iterations"},
         "fast": {"seconds": round(t32, 4), "iters": r32.iters, "converged": r32.converged,
                  "overflow": r32.overflow, "hpwl": r32.hpwl, "ms_per_iter": round(t32 / r32.iters * 1e3, 4)},
         "plain_fp64": {"seconds": round(t64, 4), "iters": r64.iters, "converged": r64.converged,
@@ -129,7 +145,7 @@ def single(args):
             # legalization (full bucket scan) and integer HPWL once per
             # 5-iteration interval, as its loop runs them
             t0 = time.perf_counter()
-            o = O.place(d, max_iters=500, field_mode=1)
+            o = O.place(d, max_iters=500, field_mode=1, stop_every_iter=int(args.stop == "every"))
             t_loop = time.perf_counter() - t0
             n_leg = (o["iters"] + 4) // 5
             t0 = time.perf_counter()
@@ -177,7 +193,7 @@ def serve(args):
         if t > args.duration:
             break
         arrivals.append((t, int(rng.integers(0, len(pool)))))
-    cfg = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
+    cfg = place_cfg(args)
     dev = ep.Device(0, hbm_budget_bytes=200 << 30)
     srv = ep.Server(dev, max_slots=128, iters_per_sync=5)
     # warm the transform cache and code paths outside the clock
@@ -218,9 +234,22 @@ def serve(args):
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p95": float(np.percentile(lat, 95)),
                        "p99": float(np.percentile(lat, 99)), "max": float(lat.max())},
         "config": {"workload": "serving: Poisson arrivals, log-uniform 1k-500k cells, 128 slots, 200 GB",
-                   "pool": len(pool), "seed": 1},
+                   "pool": len(pool), "seed": 1,
+                   "stop_check": "every iteration" if args.stop == "every" else "every 5 iterations"},
         "dtype": "f32+bf16x3-mfma+f64-reductions", "data": "synthetic (seeded generator)",
     }
+    # HPWL of every served result against the plain FP64 placer on the same
+    # pool design (one design at a time, after the serving window)
+    if not args.no_plain:
+        plain = {}
+        for k in sorted({tk[1] for tk in tickets}):
+            i = dev.upload(pool[k], "fp64")
+            plain[k] = dev.place(i, cfg, positions=False).hpwl
+            dev.free(i)
+        devs = [abs(tk[2].hpwl - plain[tk[1]]) / plain[tk[1]] for tk in tickets
+                if tk[2].status == 0 and plain[tk[1]] > 0]
+        out["hpwl_rel_dev_vs_plain_fp64"] = {"max": float(max(devs)), "mean": float(np.mean(devs)),
+                                             "designs": len(plain)}
     print(json.dumps(out), flush=True)
     dev.close()
 
@@ -297,7 +326,7 @@ def pin_one_core():
     return {"core": allowed[0], "host_cores_allowed": len(allowed), "cpu_model": model}
 
 
-def cpu_baseline(designs, budget_s):
+def cpu_baseline(designs, budget_s, every=True):
     """FP64 oracle (tests/_oracle.py, the checker) on a bounded sample of the
     batch, one core; returns designs/sec over the designs it finished."""
     host = pin_one_core()
@@ -309,7 +338,7 @@ def cpu_baseline(designs, budget_s):
     for i in order:
         if time.perf_counter() - t0 > budget_s:
             break
-        O.place(designs[i], max_iters=500, field_mode=1)
+        O.place(designs[i], max_iters=500, field_mode=1, stop_every_iter=int(every))
         done.append(i)
     dt = time.perf_counter() - t0
     return {"value": round(len(done) / dt, 4), "unit": "designs/s", "cores": 1, "kind": "port",
@@ -334,7 +363,7 @@ def main():
 
     sizes = ep.batch_sizes(args.designs)
     designs = [ep.sized_design(n, i) for i, n in enumerate(sizes)]
-    cfg = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
+    cfg = place_cfg(args)
     dev = ep.Device(0)
     ids = [dev.upload(d, "fast") for d in designs]
     # transform engine per bin shape, chosen by timing each candidate (setup,
@@ -383,6 +412,7 @@ def main():
         "config": {
             "workload": "batched: 64 designs, log-uniform 2k-100k cells, nets 1.1x, side sqrt(N/0.7), "
                         "pow2 bins 64-512, areas 1/2/4, idxst field, stop overflow<=0.10",
+            "stop_check": "every iteration" if args.stop == "every" else "every 5 iterations",
             "designs": len(ids),
             "total_cells": int(sum(d.n_mov for d in designs)),
             "total_pins": int(sum(d.n_pins for d in designs)),
@@ -396,7 +426,7 @@ def main():
     # plain FP64 GPU placer, one design at a time (reference point, not timed above)
     if not args.no_plain:
         plain_ids = [dev.upload(d, "fp64") for d in designs]
-        cfg64 = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
+        cfg64 = place_cfg(args)
         dev.place(plain_ids[0], cfg64, positions=False)  # build transforms outside the timing
         t0 = time.perf_counter()
         plain = [dev.place(i, cfg64, positions=False) for i in plain_ids]
@@ -429,6 +459,9 @@ def main():
                            "traffic": traffic}
     out["roofline"]["algorithmic_per_launch"] = work[g["work"]]
     out["roofline"]["launch_us"] = round(ms * 1e3, 2)
+    if traffic:  # what the kernels moved through the fabric (calibrated counters)
+        out["roofline"]["traffic_GBps"] = round(traffic / (ms * 1e-3) / 1e9, 1)
+        out["roofline"]["traffic_frac"] = round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     if "note" in g:
         out["roofline"]["note"] = g["note"]
     out["phase_ms_per_iter"] = {k: round(v, 4) for k, v in prof.items()}
@@ -437,11 +470,13 @@ def main():
         if gms <= 0:
             continue
         scale, unit = (1e12, "TFLOP/s") if gg["bound"] == "mfma" else (1e9, "GB/s")
+        tr = group_traffic(args.traffic_file, gg)
         out["kernel_rooflines"][name] = {"us": round(gms * 1e3, 2), unit: round(
-            work[gg["work"]] / (gms * 1e-3) / scale, 2), "traffic": group_traffic(args.traffic_file, gg)}
+            work[gg["work"]] / (gms * 1e-3) / scale, 2), "traffic": tr,
+            "traffic_GBps": round(tr / (gms * 1e-3) / 1e9, 1) if tr else None}
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(designs, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(designs, args.cpu_seconds, args.stop == "every")
     print(json.dumps(out), flush=True)
     dev.close()
 

@@ -7,12 +7,19 @@
 //   k_wa          WA wirelength gradient (wa.rs:51-101, common.rs:205-265):
 //                 one group of G lanes (G = 2..64, a power of two) per net,
 //                 pins read coalesced from padded slot arrays, max-shifted
-//                 exponentials, group reductions by cross-lane shuffles.
+//                 exponentials, group reductions by cross-lane shuffles; each
+//                 pin's gradient goes to its own slot.  In HPWL mode the
+//                 continuous HPWL (wirelength.rs:10-51) per wave.
 //   k_wa_hf       the same for nets of more than 64 pins, one block per net.
-//   k_scatter     cell -> bin overlap density (density.rs:54-74); integer bin
-//                 ranges computed in FP64 exactly as the oracle does.
-//   k_dens_final  density accumulator -> density, overflow partials
-//                 (density.rs:79-86), accumulator cleared for the next use.
+//   k_wl_cell     each cell's slots summed in ascending slot order (FP64).
+//   k_tile_count, k_tile_scan, k_tile_fill
+//                 counting sort of the cells by 8x8-bin tile, every iteration.
+//   k_scatter_pull  cell -> bin overlap density (density.rs:54-74) summed per
+//                 32x32-bin super-tile in LDS, plus overflow partials
+//                 (density.rs:79-86); integer bin ranges computed in FP64
+//                 exactly as the oracle does.
+//   k_scatter, k_dens_final  the same with global accumulation, for designs
+//                 whose footprints are too wide for the super-tile halo.
 //   k_gather      field gather (density.rs:171-206) fused with the penalty
 //                 combine and preconditioner (mod.rs:145-156) and the
 //                 Barzilai-Borwein partial sums (nesterov.rs:199-218).
@@ -21,11 +28,10 @@
 //                 convergence (mod.rs:176-195), iteration log.
 //   k_update      Nesterov step + clamp + next look-ahead (nesterov.rs:41-72,
 //                 mod.rs:105-107,169-173).
-//   k_hpwl        continuous HPWL (wirelength.rs:10-51 on continuous positions).
 //
-// Fixed-point (2^32) integer atomics make the FAST path's sums independent of
-// the order in which lanes arrive, so a design's trajectory is the same alone
-// and inside any batch.
+// Every FAST-path sum that crosses threads is integer (2^32 fixed point) or
+// in a fixed order, so a design's trajectory is the same alone and inside
+// any batch.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -301,9 +307,10 @@ __device__ inline double ov_area(const BinBox& r, int bx, int by) {
     return ox * oy;
 }
 
-// One block covers kScatCells consecutive cells of the tile order `perm`
-// (refreshed every sync interval by the tile sort below), so its cells sit
-// in a compact group of bins; when the block's bin bounding box fits
+// Scatter for designs whose widest footprint is too wide for the pull
+// scatter below.  One block covers kScatCells consecutive cells of this
+// iteration's tile order `perm`, so its cells sit in a compact group of
+// bins; when the block's bin bounding box fits
 // kScatBins the contributions are summed in LDS and flushed with one global
 // atomic per touched bin, otherwise each contribution goes to global memory
 // directly.  All sums are fixed point (FAST), so neither the order nor the

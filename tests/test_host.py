@@ -187,3 +187,38 @@ def test_design_bytes_sizes_an_arena():
     b64 = d.nbytes("fp64")
     b32 = d.nbytes("fast")
     assert b64 > b32 > 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the placer runs on one GPU; bench.py says so instead of measuring
+    import subprocess
+    import sys as _sys
+    r = subprocess.run([_sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stdout
+
+
+def test_bench_roofline_inputs():
+    # algorithmic work per design follows the stated per-unit figures, and
+    # the traffic of a group adds its kernels' calibrated counter bytes
+    import json
+    import sys as _sys
+    _sys.path.insert(0, str(ROOT))
+    import bench
+    d = ep.sized_design(3000, 1)
+    w = bench.algorithmic_work([d])
+    N, F, E, P, B2 = d.n_mov, d.n_fixed, d.n_nets, d.n_pins, d.Bx * d.By
+    assert w["wa"] == 4 * (E + 1) + 4 * P + 16 * N + 8 * F
+    assert w["scatter"] == 12 * N + 4 * B2
+    assert w["gather+update"] == 60 * N + 8 * B2
+    assert w["dct_flops"] == 36 * d.Bx ** 3  # square grid: 18 Bx By (Bx + By)
+    t = {"fetch_factor": 2.0, "write_factor": 1.0,
+         "kernels": {"void ep::k_wa<float, true, false>(ep::BatchDev, int)": {"fetch_kib": 10.0, "write_kib": 4.0},
+                     "void ep::k_wl_cell<float>(ep::BatchDev)": {"fetch_kib": 1.0, "write_kib": 1.0}}}
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as f:
+        json.dump(t, f)
+    assert bench.group_traffic(f.name, bench.KERNEL_GROUPS["wa"]) == 1024 * (24 + 3)
+    assert bench.group_traffic(f.name, bench.KERNEL_GROUPS["gather+update"]) is None
+    assert bench.group_traffic("/nonexistent.json", bench.KERNEL_GROUPS["wa"]) is None
