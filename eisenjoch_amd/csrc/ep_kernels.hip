@@ -193,10 +193,37 @@ __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
     }
 }
 
-// Nets of more than 64 pins: one block per net, three passes over its pins.
+// Nets of more than 64 pins: one block of kHfThreads per net, three passes
+// over its pins (extremes, exponential sums, gradients).  A net of up to
+// kHfThreads * kHfCache pins keeps each thread's pin positions in registers
+// after the first pass; larger nets read them again.  The block reductions
+// run over the waves in a fixed order.
+constexpr int kHfThreads = 512;
+constexpr int kHfWaves = kHfThreads / kWave;
+constexpr int kHfCache = 10;
+
+template <int NV>
+__device__ inline void hf_block_sum(double (&v)[NV], double* sh, double* out) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double x = v[q];
+        for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, kWave);
+        if (lane == 0) sh[w * NV + q] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double t = 0.0;
+        for (int k = 0; k < kHfWaves; k++) t += sh[k * NV + threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
 template <typename T, bool FIX, bool HPWL>
-__global__ __launch_bounds__(256) void k_wa_hf(BatchDev b, int final_pass) {
-    __shared__ double sh[4 * 8];
+__global__ __launch_bounds__(kHfThreads) void k_wa_hf(BatchDev b, int final_pass) {
+    __shared__ double sh[kHfWaves * 8];
+    __shared__ double tot[8];
     const int d = find_blk(b.pfx_hf, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     const DevState* st = D.st;
@@ -208,18 +235,34 @@ __global__ __launch_bounds__(256) void k_wa_hf(BatchDev b, int final_pass) {
     const int e = blockIdx.x - b.pfx_hf[d];
     if (e >= D.n_hf) return;
     const uint32_t p0 = D.hf_off[e], p1 = D.hf_off[e + 1];
+    const bool cached = p1 - p0 <= (uint32_t)(kHfThreads * kHfCache);
     const T big = Lim<T>::big();
+    T cx[kHfCache], cy[kHfCache];
     double mm[4] = {-(double)big, -(double)big, -(double)big, -(double)big};  // xmax, -xmin, ymax, -ymin
-    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        T px, py;
-        pin_pos<T, HPWL>(D, D.hf_pins[p], px, py);
+    auto extremes = [&](T px, T py) {
         mm[0] = fmax(mm[0], (double)px);
         mm[1] = fmax(mm[1], -(double)px);
         mm[2] = fmax(mm[2], (double)py);
         mm[3] = fmax(mm[3], -(double)py);
+    };
+    if (cached) {
+#pragma unroll
+        for (int q = 0; q < kHfCache; q++) {
+            const uint32_t p = p0 + threadIdx.x + (uint32_t)q * kHfThreads;
+            cx[q] = cy[q] = T(0);
+            if (p < p1) {
+                pin_pos<T, HPWL>(D, D.hf_pins[p], cx[q], cy[q]);
+                extremes(cx[q], cy[q]);
+            }
+        }
+    } else {
+        for (uint32_t p = p0 + threadIdx.x; p < p1; p += kHfThreads) {
+            T px, py;
+            pin_pos<T, HPWL>(D, D.hf_pins[p], px, py);
+            extremes(px, py);
+        }
     }
-    // block max
-    {
+    {  // block max, fixed order
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -229,8 +272,11 @@ __global__ __launch_bounds__(256) void k_wa_hf(BatchDev b, int final_pass) {
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            mm[q] = fmax(fmax(sh[q], sh[4 + q]), fmax(sh[8 + q], sh[12 + q]));
+        for (int q = 0; q < 4; q++) {
+            double x = sh[q];
+            for (int k = 1; k < kHfWaves; k++) x = fmax(x, sh[k * 4 + q]);
+            mm[q] = x;
+        }
         __syncthreads();
     }
     const T xmx = (T)mm[0], xmn = (T)(-mm[1]), ymx = (T)mm[2], ymn = (T)(-mm[3]);
@@ -241,34 +287,48 @@ __global__ __launch_bounds__(256) void k_wa_hf(BatchDev b, int final_pass) {
     }
     const T a = (T)D.cfg.wl_coeff;
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        T px, py;
-        pin_pos<T, HPWL>(D, D.hf_pins[p], px, py);
+    auto sums = [&](T px, T py) {
         T ux = px - xmx, vx = px - xmn, uy = py - ymx, vy = py - ymn;
         T ex = fexp(a * ux), fx = fexp(-a * vx), ey = fexp(a * uy), fy = fexp(-a * vy);
         s[0] += ex; s[1] += ux * ex; s[2] += fx; s[3] += vx * fx;
         s[4] += ey; s[5] += uy * ey; s[6] += fy; s[7] += vy * fy;
-    }
-    block_sum<8>(s, sh);
-    __syncthreads();
-    __shared__ double tot[8];
-    if (threadIdx.x == 0)
+    };
+    if (cached) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) tot[q] = s[q];
-    __syncthreads();
+        for (int q = 0; q < kHfCache; q++)
+            if (p0 + threadIdx.x + (uint32_t)q * kHfThreads < p1) sums(cx[q], cy[q]);
+    } else {
+        for (uint32_t p = p0 + threadIdx.x; p < p1; p += kHfThreads) {
+            T px, py;
+            pin_pos<T, HPWL>(D, D.hf_pins[p], px, py);
+            sums(px, py);
+        }
+    }
+    hf_block_sum<8>(s, sh, tot);
     T S[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) S[q] = (T)tot[q];
-    for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        uint32_t c = D.hf_pins[p];
-        if (c >= (uint32_t)D.n_mov) continue;
-        T px, py;
-        pin_pos<T, HPWL>(D, c, px, py);
+    auto grad = [&](uint32_t p, T px, T py) {
         T ux = px - xmx, vx = px - xmn, uy = py - ymx, vy = py - ymn;
         T ex = fexp(a * ux), fx = fexp(-a * vx), ey = fexp(a * uy), fy = fexp(-a * vy);
         T gx = wa_pin<T>(a, ux, ex, S[0], S[1], vx, fx, S[2], S[3]);
         T gy = wa_pin<T>(a, uy, ey, S[4], S[5], vy, fy, S[6], S[7]);
         store_grad<T>(D, (uint32_t)D.n_wa_waves * kWave + p, gx, gy);
+    };
+    if (cached) {
+#pragma unroll
+        for (int q = 0; q < kHfCache; q++) {
+            const uint32_t p = p0 + threadIdx.x + (uint32_t)q * kHfThreads;
+            if (p < p1 && D.hf_pins[p] < (uint32_t)D.n_mov) grad(p, cx[q], cy[q]);
+        }
+    } else {
+        for (uint32_t p = p0 + threadIdx.x; p < p1; p += kHfThreads) {
+            const uint32_t c = D.hf_pins[p];
+            if (c >= (uint32_t)D.n_mov) continue;
+            T px, py;
+            pin_pos<T, HPWL>(D, c, px, py);
+            grad(p, px, py);
+        }
     }
 }
 
@@ -1077,9 +1137,9 @@ void launch_wa(const BatchDev& b, int prec, int wa_blocks, int hf_blocks, hipStr
     }
     if (hf_blocks > 0) {
         if (prec == EP_PREC_FP64)
-            hipLaunchKernelGGL((k_wa_hf<double, false, false>), dim3(hf_blocks), dim3(256), 0, s, b, 0);
+            hipLaunchKernelGGL((k_wa_hf<double, false, false>), dim3(hf_blocks), dim3(kHfThreads), 0, s, b, 0);
         else
-            hipLaunchKernelGGL((k_wa_hf<float, true, false>), dim3(hf_blocks), dim3(256), 0, s, b, 0);
+            hipLaunchKernelGGL((k_wa_hf<float, true, false>), dim3(hf_blocks), dim3(kHfThreads), 0, s, b, 0);
     }
 }
 
@@ -1093,9 +1153,9 @@ void launch_hpwl(const BatchDev& b, int prec, int wa_blocks, int hf_blocks, int 
     }
     if (hf_blocks > 0) {
         if (prec == EP_PREC_FP64)
-            hipLaunchKernelGGL((k_wa_hf<double, false, true>), dim3(hf_blocks), dim3(256), 0, s, b, final_pass);
+            hipLaunchKernelGGL((k_wa_hf<double, false, true>), dim3(hf_blocks), dim3(kHfThreads), 0, s, b, final_pass);
         else
-            hipLaunchKernelGGL((k_wa_hf<float, true, true>), dim3(hf_blocks), dim3(256), 0, s, b, final_pass);
+            hipLaunchKernelGGL((k_wa_hf<float, true, true>), dim3(hf_blocks), dim3(kHfThreads), 0, s, b, final_pass);
     }
 }
 
