@@ -22,6 +22,7 @@ constexpr int kBinParts = 2;         // partial sums per bin block
 constexpr double kFixScale = 4294967296.0;  // 2^32 fixed point for deterministic sums
 constexpr int kScatCells = 1024;     // cells per scatter block
 constexpr int kScatBins = 6144;      // LDS accumulation window of a scatter block (bins)
+constexpr int kHfStat = 16;      // per high-fanout net: 4 extremes, 8 sums, weight, pad
 constexpr int kSplitMinK = 512;   // K from which transform products are split
 constexpr int kSplitSlice = 128;  // K per slice of a split product
 
@@ -86,7 +87,10 @@ struct DesignDev {
     uint32_t* cs_off;          // cell -> slot CSR [n_mov + 1]
     double* gwl;               // [n_mov][2] WA gradient per cell (its slots summed in FP64)
     const void* slot_w;        // T [n_slots]: each slot's net weight, or null (all nets weigh 1)
-    uint32_t* cs_idx;          // slots of each movable cell, ascending
+    uint32_t* cs_idx;          // slots of each movable cell, ascending (nets of <= 64 pins)
+    uint32_t* chf_off;         // cell -> high-fanout-net CSR [n_mov + 1]
+    uint32_t* chf_net;         // high-fanout nets of each cell, ascending
+    void* hf_stat;             // T [n_hf][kHfStat]: extremes, exponential sums, weight
     uint32_t* perm[2];         // cells in Morton order of their lower-left bin tile; the
                                // sort of iteration k reads perm[k&1] and writes perm[(k+1)&1]
     uint32_t* keys;            // tile key of perm[k&1][j], by position j

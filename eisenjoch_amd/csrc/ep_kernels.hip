@@ -193,8 +193,9 @@ __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
     }
 }
 
-// Nets of more than 64 pins: one block of kHfThreads per net, three passes
-// over its pins (extremes, exponential sums, gradients).  A net of up to
+// Nets of more than 64 pins: one block of kHfThreads per net, two passes
+// over its pins (extremes, then exponential sums), and the net's statistics
+// written for the cells to form their pins' gradients from.  A net of up to
 // kHfThreads * kHfCache pins keeps each thread's pin positions in registers
 // after the first pass; larger nets read them again.  The block reductions
 // run over the waves in a fixed order.
@@ -305,30 +306,14 @@ __global__ __launch_bounds__(kHfThreads) void k_wa_hf(BatchDev b, int final_pass
         }
     }
     hf_block_sum<8>(s, sh, tot);
-    T S[8];
+    // the net's extremes, sums and weight; each cell derives its pins'
+    // gradients from them (gwl_of), so no per-pin slot is written
+    if (threadIdx.x == 0) {
+        T* out = (T*)D.hf_stat + (size_t)e * kHfStat;
+        out[0] = xmx; out[1] = xmn; out[2] = ymx; out[3] = ymn;
 #pragma unroll
-    for (int q = 0; q < 8; q++) S[q] = (T)tot[q];
-    auto grad = [&](uint32_t p, T px, T py) {
-        T ux = px - xmx, vx = px - xmn, uy = py - ymx, vy = py - ymn;
-        T ex = fexp(a * ux), fx = fexp(-a * vx), ey = fexp(a * uy), fy = fexp(-a * vy);
-        T gx = wa_pin<T>(a, ux, ex, S[0], S[1], vx, fx, S[2], S[3]);
-        T gy = wa_pin<T>(a, uy, ey, S[4], S[5], vy, fy, S[6], S[7]);
-        store_grad<T>(D, (uint32_t)D.n_wa_waves * kWave + p, gx, gy);
-    };
-    if (cached) {
-#pragma unroll
-        for (int q = 0; q < kHfCache; q++) {
-            const uint32_t p = p0 + threadIdx.x + (uint32_t)q * kHfThreads;
-            if (p < p1 && D.hf_pins[p] < (uint32_t)D.n_mov) grad(p, cx[q], cy[q]);
-        }
-    } else {
-        for (uint32_t p = p0 + threadIdx.x; p < p1; p += kHfThreads) {
-            const uint32_t c = D.hf_pins[p];
-            if (c >= (uint32_t)D.n_mov) continue;
-            T px, py;
-            pin_pos<T, HPWL>(D, c, px, py);
-            grad(p, px, py);
-        }
+        for (int q = 0; q < 8; q++) out[4 + q] = (T)tot[q];
+        out[12] = D.slot_w ? ((const T*)D.slot_w)[(size_t)D.n_wa_waves * kWave + p0] : T(1);
     }
 }
 
@@ -698,10 +683,12 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
 
 // ------------------------------------------------------------ gather ----
 
-// WA gradient of cell i: its pin slots summed in ascending slot order.  The
-// slot indices and slot values of four pins are loaded before any is added,
-// so four dependent load pairs are in flight at once; the additions keep
-// the sequential order.
+// WA gradient of cell i: its pin slots (nets of <= 64 pins) summed in
+// ascending slot order, then its pins on high-fanout nets, in ascending net
+// order, each formed from the net's statistics exactly as a slot would hold
+// it.  The slot indices and slot values of four pins are loaded before any
+// is added, so four dependent load pairs are in flight at once; the
+// additions keep the sequential order.
 template <typename T>
 __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy) {
     typedef typename Vec2<T>::type V;
@@ -727,6 +714,24 @@ __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy)
         sx += (double)v0.x; sy += (double)v0.y;
         if (rest > 1) { sx += (double)v1.x; sy += (double)v1.y; }
         if (rest > 2) { sx += (double)v2.x; sy += (double)v2.y; }
+    }
+    const uint32_t h0 = D.chf_off[i], h1 = D.chf_off[i + 1];
+    if (h1 > h0) {
+        const V l = ((const V*)D.look)[i];
+        const T a = (T)D.cfg.wl_coeff;
+        for (uint32_t k = h0; k < h1; k++) {
+            const T* st = (const T*)D.hf_stat + (size_t)D.chf_net[k] * kHfStat;
+            const T ux = l.x - st[0], vx = l.x - st[1], uy = l.y - st[2], vy = l.y - st[3];
+            const T ex = fexp(a * ux), fx = fexp(-a * vx), ey = fexp(a * uy), fy = fexp(-a * vy);
+            T gxp = wa_pin<T>(a, ux, ex, st[4], st[5], vx, fx, st[6], st[7]);
+            T gyp = wa_pin<T>(a, uy, ey, st[8], st[9], vy, fy, st[10], st[11]);
+            if (D.slot_w) {  // net weight, common.rs:246-261
+                gxp = st[12] * gxp;
+                gyp = st[12] * gyp;
+            }
+            sx += (double)gxp;
+            sy += (double)gyp;
+        }
     }
     gx = sx;
     gy = sy;

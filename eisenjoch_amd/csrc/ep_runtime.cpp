@@ -184,22 +184,33 @@ static void build_layout(const ep_design* d, NetLayout& L) {
         waves += (int)((L.slots.size() - start) / kWave);
     }
     L.bucket_wave_start[kMaxBuckets] = waves;
-    // cell -> slot CSR (counting sort; slots visited in ascending order)
+    // cell -> slot CSR over the slots of nets of <= 64 pins (counting sort;
+    // slots visited in ascending order)
     const size_t nws = L.slots.size();
     L.cs_off.assign((size_t)nm + 1, 0);
-    auto slot_cell = [&](size_t s) -> uint32_t { return s < nws ? L.slots[s] : L.hf_pins[s - nws]; };
-    const size_t ns = nws + L.hf_pins.size();
-    for (size_t s = 0; s < ns; s++) {
-        uint32_t c = slot_cell(s);
+    for (size_t s = 0; s < nws; s++) {
+        uint32_t c = L.slots[s];
         if (c != kPad && c < nm) L.cs_off[c + 1]++;
     }
     for (size_t i = 0; i < nm; i++) L.cs_off[i + 1] += L.cs_off[i];
     L.cs_idx.resize(L.cs_off[nm]);
     std::vector<uint32_t> fillp(L.cs_off.begin(), L.cs_off.end() - 1);
-    for (size_t s = 0; s < ns; s++) {
-        uint32_t c = slot_cell(s);
+    for (size_t s = 0; s < nws; s++) {
+        uint32_t c = L.slots[s];
         if (c != kPad && c < nm) L.cs_idx[fillp[c]++] = (uint32_t)s;
     }
+    // cell -> high-fanout-net CSR (nets ascending, a net once per pin of the cell)
+    L.chf_off.assign((size_t)nm + 1, 0);
+    const size_t nhf = L.hf_off.size() - 1;
+    for (size_t h = 0; h < nhf; h++)
+        for (uint32_t p = L.hf_off[h]; p < L.hf_off[h + 1]; p++)
+            if (L.hf_pins[p] < nm) L.chf_off[L.hf_pins[p] + 1]++;
+    for (size_t i = 0; i < nm; i++) L.chf_off[i + 1] += L.chf_off[i];
+    L.chf_net.resize(L.chf_off[nm]);
+    std::vector<uint32_t> fillh(L.chf_off.begin(), L.chf_off.end() - 1);
+    for (size_t h = 0; h < nhf; h++)
+        for (uint32_t p = L.hf_off[h]; p < L.hf_off[h + 1]; p++)
+            if (L.hf_pins[p] < nm) L.chf_net[fillh[L.hf_pins[p]]++] = (uint32_t)h;
 }
 
 static int validate_design(Ctx* c, const ep_design* d) {
@@ -266,6 +277,9 @@ static size_t plan_arena(Design& D, char* base) {
     G.cs_off = (uint32_t*)P((N + 1) * 4);
     G.gwl = (double*)P(N * 16);
     G.cs_idx = (uint32_t*)P(D.cs_count * 4);
+    G.chf_off = (uint32_t*)P((N + 1) * 4);
+    G.chf_net = (uint32_t*)P(D.chf_count * 4);
+    G.hf_stat = P((size_t)G.n_hf * kHfStat * T);
     G.perm[0] = (uint32_t*)P(N * 4);
     G.perm[1] = (uint32_t*)P(N * 4);
     G.keys = (uint32_t*)P(N * 4);
@@ -354,6 +368,7 @@ static void fill_desc_sizes(Design& D, const ep_design* d, const NetLayout& L) {
         }
     }
     D.cs_count = L.cs_off.empty() ? 0 : L.cs_off.back();
+    D.chf_count = L.chf_off.empty() ? 0 : L.chf_off.back();
     D.weighted = d->net_weight != nullptr;
 }
 
@@ -624,6 +639,8 @@ int commit_design(Ctx* c, Prepared& P, uint32_t* id_out) {
     if (!rc) rc = copy_in(c, D.dev.hf_pins, L.hf_pins.data(), L.hf_pins.size() * 4);
     if (!rc) rc = copy_in(c, D.dev.cs_off, L.cs_off.data(), L.cs_off.size() * 4);
     if (!rc) rc = copy_in(c, D.dev.cs_idx, L.cs_idx.data(), L.cs_idx.size() * 4);
+    if (!rc) rc = copy_in(c, D.dev.chf_off, L.chf_off.data(), L.chf_off.size() * 4);
+    if (!rc) rc = copy_in(c, D.dev.chf_net, L.chf_net.data(), L.chf_net.size() * 4);
     if (!rc && D.dev.slot_w) rc = copy_in(c, (void*)D.dev.slot_w, P.slot_w.data(), P.slot_w.size());
     if (rc) {
         hipStreamSynchronize(c->stream);

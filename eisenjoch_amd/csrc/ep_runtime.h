@@ -44,7 +44,8 @@ struct Design {
     void* arena = nullptr;
     size_t bytes = 0;
     size_t hf_pin_count = 0;
-    size_t cs_count = 0;            // movable pins (cell -> slot CSR entries)
+    size_t cs_count = 0;            // movable pins of nets of <= 64 pins (cell -> slot CSR entries)
+    size_t chf_count = 0;           // movable pins of high-fanout nets
     bool weighted = false;          // per-net weights given (slot_w allocated)
     DesignDev dev{};
 };
@@ -122,6 +123,7 @@ struct NetLayout {
     std::vector<uint32_t> cs_off, cs_idx;
     std::vector<int32_t> slot_net;  // net of each slot of `slots` (-1: padding)
     std::vector<int32_t> hf_net;    // net of each high-fanout net
+    std::vector<uint32_t> chf_off, chf_net;  // cell -> high-fanout nets it has pins on
 };
 
 // Everything an upload computes on the host (layout, sizes, the initial
