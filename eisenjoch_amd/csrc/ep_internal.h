@@ -100,7 +100,7 @@ struct DesignDev {
     float* sorted_hs[2];       // half sides in the order of perm[0] / perm[1]
     uint32_t* tcount;          // [n_keys] cells per tile key (zero between sorts)
     uint32_t* tstart;          // [n_keys + 1] first position of each key in perm
-    uint32_t* tcur;            // [n_keys] fill cursors
+    uint32_t* tofs;            // [n_mov] offset of perm[k&1][j] within its tile, by j
     void* pinw;                // T[n_mov]
     float* hs;                 // half side of the footprint [n_mov]
     void* fixed[2];            // T[n_fixed]

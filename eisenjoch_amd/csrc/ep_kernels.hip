@@ -491,11 +491,12 @@ __device__ inline uint32_t wave_agg_add(uint32_t* ctr, uint32_t key, bool valid)
 }
 
 // Counting sort of the cells by tile key, every iteration: count, scan, fill.
-// Both passes walk the cells in the previous iteration's order, in which
-// neighbours mostly share a tile, so a wave sees few distinct keys.  The
-// count pass gathers each cell's position once (the half sides are kept in
-// the order of each permutation by the fill that wrote it) and stages both by
-// position, so the fill pass reads only contiguous arrays.
+// The count pass walks the cells in the previous iteration's order, in which
+// neighbours mostly share a tile, so a wave sees few distinct keys; it
+// gathers each cell's position once (the half sides are kept in the order of
+// each permutation by the fill that wrote it), stages key, position and the
+// cell's offset inside its tile by position, so the fill pass reads only
+// contiguous arrays and places each cell at tstart[key] + offset.
 template <typename T>
 __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
     const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
@@ -517,15 +518,18 @@ __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
         sxy[2 * (size_t)k + 1] = y;
         D.stage_hs[k] = hs;
     }
-    wave_agg_add(D.tcount, key, valid);
+    // the counter's old value is this cell's offset inside its tile, so
+    // the fill pass needs no atomics of its own
+    const uint32_t ofs = wave_agg_add(D.tcount, key, valid);
+    if (valid) D.tofs[k] = ofs;
 }
 
 // One block per design: tstart = exclusive scan of the counts (with the
-// total at n_keys), cursors = tstart, counts cleared for the next sort.
+// total at n_keys), counts cleared for the next sort.
 // Each thread owns a contiguous run of keys (16-byte vectors when the key
 // count is a multiple of 1,024, as every power of four from 1,024 up is);
 // the run totals are scanned with wave shuffles and one LDS step across the
-// four waves.  The three arrays are distinct buffers (restrict), so a run's
+// four waves.  The two arrays are distinct buffers (restrict), so a run's
 // loads are all issued before its stores.
 __global__ __launch_bounds__(256) void k_tile_scan(BatchDev b) {
     __shared__ uint32_t wsum[4];
@@ -533,7 +537,6 @@ __global__ __launch_bounds__(256) void k_tile_scan(BatchDev b) {
     if (D.st->done) return;
     uint32_t* __restrict__ cnt = D.tcount;
     uint32_t* __restrict__ ts = D.tstart;
-    uint32_t* __restrict__ tc = D.tcur;
     const int n = D.n_keys;
     const bool vec = (n & 1023) == 0;
     const int per = (n + 255) / 256;
@@ -568,14 +571,12 @@ __global__ __launch_bounds__(256) void k_tile_scan(BatchDev b) {
             st.z = run; run += c.z;
             st.w = run; run += c.w;
             *(uint4*)&ts[k] = st;
-            *(uint4*)&tc[k] = st;
             *(uint4*)&cnt[k] = make_uint4(0u, 0u, 0u, 0u);
         }
     } else {
         for (int k = lo; k < hi; k++) {
             const uint32_t c = cnt[k];
             ts[k] = run;
-            tc[k] = run;
             cnt[k] = 0u;
             run += c;
         }
@@ -591,8 +592,7 @@ __global__ __launch_bounds__(256) void k_tile_fill(BatchDev b) {
     const bool valid = k < D.n_mov;
     const int par = D.st->it & 1;
     const uint32_t i = valid ? D.perm[par][k] : 0u;
-    const uint32_t key = valid ? D.keys[k] : 0u;
-    const uint32_t pos = wave_agg_add(D.tcur, key, valid);
+    const uint32_t pos = valid ? D.tstart[D.keys[k]] + D.tofs[k] : 0u;
     if (valid && pos < (uint32_t)D.n_mov) {
         const T* gxy = (const T*)D.stage_xy;
         D.perm[par ^ 1][pos] = i;

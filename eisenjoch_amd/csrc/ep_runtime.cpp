@@ -290,7 +290,7 @@ static size_t plan_arena(Design& D, char* base) {
     G.sorted_hs[1] = (float*)P(N * 4);
     G.tcount = (uint32_t*)P((size_t)G.n_keys * 4);
     G.tstart = (uint32_t*)P(((size_t)G.n_keys + 1) * 4);
-    G.tcur = (uint32_t*)P((size_t)G.n_keys * 4);
+    G.tofs = (uint32_t*)P(N * 4);
     G.pinw = P(N * T);
     G.hs = (float*)P(N * 4);
     G.slot_w = D.weighted ? P((size_t)G.n_slots * T) : nullptr;
