@@ -24,6 +24,24 @@ namespace ep {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Operand and output pointers come from job descriptors in memory; as
+// generic pointers they would compile to flat accesses, which count on the
+// LDS/scalar counter too, so every wait for an LDS read or a descriptor field
+// would also drain the operand loads in flight.  They are cast to the global
+// address space instead.
+#define EP_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+__device__ inline const EP_GLOBAL T* gl(const void* p) {
+    return (const EP_GLOBAL T*)p;
+}
+
+template <typename T>
+__device__ inline EP_GLOBAL T* gw(void* p) {
+    return (EP_GLOBAL T*)p;
+}
 
 __device__ inline int find_job(const GemmJob* jobs, int nj, int blk) {
     int lo = 0, hi = nj;
@@ -42,14 +60,14 @@ __device__ inline void epi_store(const GemmJob& J, const DesignDev& D, int row, 
                                  size_t po = 0) {
     if (row >= J.M || col >= J.N) return;
     if (J.epi == kEpiStore) {
-        ((T*)J.C)[po + (size_t)row * J.N + col] = (T)v;
+        gw<T>(J.C)[po + (size_t)row * J.N + col] = (T)v;
     } else if (J.epi == kEpiStoreT) {
-        ((T*)J.C)[po + (size_t)col * J.M + row] = (T)v;
+        gw<T>(J.C)[po + (size_t)col * J.M + row] = (T)v;
     } else if (J.epi == kEpiStore2) {
-        ((T*)J.C)[po + 2 * ((size_t)row * J.N + col)] = (T)v;
+        gw<T>(J.C)[po + 2 * ((size_t)row * J.N + col)] = (T)v;
     } else {
-        const T* sx = (const T*)D.spec_x;  // [kx] -> (w, 2(1 - cos w))
-        const T* sy = (const T*)D.spec_y;
+        const EP_GLOBAL T* sx = gl<T>(D.spec_x);  // [kx] -> (w, 2(1 - cos w))
+        const EP_GLOBAL T* sy = gl<T>(D.spec_y);
         double s = 4.0 / ((double)D.Bx * (double)D.By);
         if (col == 0) s *= 0.5;
         if (row == 0) s *= 0.5;
@@ -63,8 +81,8 @@ __device__ inline void epi_store(const GemmJob& J, const DesignDev& D, int row, 
             ex = phi * wx;
             ey = phi * wy;
         }
-        ((T*)J.C)[po + (size_t)row * J.N + col] = (T)ex;
-        ((T*)J.C2)[po + (size_t)row * J.N + col] = (T)ey;
+        gw<T>(J.C)[po + (size_t)row * J.N + col] = (T)ex;
+        gw<T>(J.C2)[po + (size_t)row * J.N + col] = (T)ey;
     }
 }
 
@@ -81,8 +99,8 @@ __global__ __launch_bounds__(256) void k_gemm_fma(const GemmJob* jobs, int nj, c
     const int t = blockIdx.x - J.tile_start;
     const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
     const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-    const T* A = (const T*)J.A;
-    const T* B = (const T*)J.B;
+    const EP_GLOBAL T* A = gl<T>(J.A);
+    const EP_GLOBAL T* B = gl<T>(J.B);
     T acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -127,63 +145,87 @@ __device__ inline void split8(const float* v, bf16x8& hi, bf16x8& lo) {
     }
 }
 
-// One thread's share of a 64 x TK operand tile: TK / 32 chunks of 8
-// consecutive K values of one row (chunk c at K offset 8 (tid % 4) + 32 c),
-// held as BF16 hi/lo in registers between the global load and the LDS store
-// (so the next stage's loads overlap this stage's MFMAs).
+// One thread's share of one K stage (64 rows x TK) of each operand: TK / 32
+// chunks of 8 consecutive K values of row tid / 4 (chunk c at K offset
+// 8 (tid % 4) + 32 c).  The data operand stays FP32 in registers until the
+// stage is written to LDS, where it is split into BF16 hi/lo: a wait for the
+// load then sits where the stage is consumed, not where it is issued.  The
+// constant operand arrives pre-split.
 template <int TK>
-struct Frag {
-    bf16x8 hi[TK / 32], lo[TK / 32];
+struct Stage {
+    f32x4 d0[TK / 32], d1[TK / 32];   // data: K values 0-3 and 4-7 of each chunk
+    bf16x8 hi[TK / 32], lo[TK / 32];  // constant
 };
 
 // A data operand written by a split pass is the sum of its `parts` planes
-// (pstride elements apart), added in plane order.
-template <int TK>
-__device__ inline Frag<TK> load_frag(int kind, const void* p, const void* plo, int r0, int k0, int K,
-                                     int parts, int64_t pstride) {
-    const int tid = threadIdx.x;
-    const int row = tid >> 2, kc = (tid & 3) * 8;
-    Frag<TK> f;
+// (pstride elements apart), added in plane order (SUM only).
+template <int TK, bool SUM>
+__device__ inline void load_stage(Stage<TK>& s, const float* pd, int rd, const __bf16* ch,
+                                  const __bf16* cl, int rc, int k0, int K, int parts,
+                                  int64_t pstride) {
+    const int row = threadIdx.x >> 2, kc = (threadIdx.x & 3) * 8;
 #pragma unroll
     for (int c = 0; c < TK / 32; c++) {
-        const size_t off = (size_t)(r0 + row) * K + k0 + kc + 32 * c;
-        if (kind == 0) {
-            const float4* src = (const float4*)((const float*)p + off);
-            float4 a = src[0], b = src[1];
+        const size_t od = (size_t)(rd + row) * K + k0 + kc + 32 * c;
+        const EP_GLOBAL f32x4* src = gl<f32x4>(pd + od);
+        s.d0[c] = src[0];
+        s.d1[c] = src[1];
+        if (SUM)
             for (int q = 1; q < parts; q++) {
-                const float4* sq = (const float4*)((const float*)p + off + q * pstride);
-                const float4 a2 = sq[0], b2 = sq[1];
-                a.x += a2.x; a.y += a2.y; a.z += a2.z; a.w += a2.w;
-                b.x += b2.x; b.y += b2.y; b.z += b2.z; b.w += b2.w;
+                const EP_GLOBAL f32x4* sq = gl<f32x4>(pd + od + q * pstride);
+                s.d0[c] += sq[0];
+                s.d1[c] += sq[1];
             }
-            const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-            split8(v, f.hi[c], f.lo[c]);
-        } else {
-            f.hi[c] = *(const bf16x8*)((const __bf16*)p + off);
-            f.lo[c] = *(const bf16x8*)((const __bf16*)plo + off);
-        }
+        const size_t oc = (size_t)(rc + row) * K + k0 + kc + 32 * c;
+        s.hi[c] = *gl<bf16x8>(ch + oc);
+        s.lo[c] = *gl<bf16x8>(cl + oc);
     }
-    return f;
 }
 
 template <int TK>
-__device__ inline void store_frag(__bf16 (*s)[64][TK + 8], const Frag<TK>& f) {
-    const int tid = threadIdx.x;
-    const int row = tid >> 2, kc = (tid & 3) * 8;
+__device__ inline void put_stage(__bf16 (*sd)[64][TK + 8], __bf16 (*sc)[64][TK + 8],
+                                 const Stage<TK>& s) {
+    const int row = threadIdx.x >> 2, kc = (threadIdx.x & 3) * 8;
 #pragma unroll
     for (int c = 0; c < TK / 32; c++) {
-        *(bf16x8*)&s[0][row][kc + 32 * c] = f.hi[c];
-        *(bf16x8*)&s[1][row][kc + 32 * c] = f.lo[c];
+        const float v[8] = {s.d0[c][0], s.d0[c][1], s.d0[c][2], s.d0[c][3],
+                            s.d1[c][0], s.d1[c][1], s.d1[c][2], s.d1[c][3]};
+        bf16x8 hi, lo;
+        split8(v, hi, lo);
+        *(bf16x8*)&sd[0][row][kc + 32 * c] = hi;
+        *(bf16x8*)&sd[1][row][kc + 32 * c] = lo;
+        *(bf16x8*)&sc[0][row][kc + 32 * c] = s.hi[c];
+        *(bf16x8*)&sc[1][row][kc + 32 * c] = s.lo[c];
+    }
+}
+
+// The three split products hi*hi + hi*lo + lo*hi of one LDS stage into the
+// wave's 32 x 32 quadrant.
+template <int TK>
+__device__ inline void mma_stage(f32x16& acc, __bf16 (*sa)[64][TK + 8], __bf16 (*sb)[64][TK + 8],
+                                 int ra, int rb, int kh) {
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 16) {
+        const bf16x8 ah = *(const bf16x8*)&sa[0][ra][kk + kh];
+        const bf16x8 al = *(const bf16x8*)&sa[1][ra][kk + kh];
+        const bf16x8 bh = *(const bf16x8*)&sb[0][rb][kk + kh];
+        const bf16x8 bl = *(const bf16x8*)&sb[1][rb][kk + kh];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
     }
 }
 
 // 64 x 64 output tile per block, four waves each owning a 32 x 32 quadrant,
-// K in stages of TK through two LDS buffers (rows padded by 8 BF16), with
-// the global loads two stages ahead of the MFMAs: the chain of stages bounds
-// a large transform, and a stage otherwise waits a whole load latency.  TK =
-// 64 halves the chain at twice the LDS (half the resident blocks).
+// K in stages of TK through two LDS buffers (rows padded by 8 BF16).  Two
+// register sets alternate (the loop is unrolled by two, so no register copy
+// waits on a load in flight): while a stage is multiplied, the loads of the
+// next two stages are outstanding.  Past the last stage the loads repeat the
+// last stage (in bounds, unused).  TK = 64 halves the chain of stages at
+// twice the LDS (half the resident blocks).  ADATA: the data operand is A
+// (F1, I1), else B (F2, I2).
 // Requires M, N multiples of 64 and K a multiple of TK (checked on the host).
-template <int TK>
+template <int TK, bool ADATA, bool SUM>
 __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, const DesignDev* dd) {
     __shared__ __bf16 sA[2][2][64][TK + 8];  // [buffer][hi/lo][row][k]
     __shared__ __bf16 sB[2][2][64][TK + 8];
@@ -199,58 +241,50 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, 
     const int kper = J.K / J.ksplit, kbeg = ks * kper;
     const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int wm = wv >> 1, wn = wv & 1;
-    const void* pa = J.a_kind ? J.A_hi : J.A;
-    const void* pb = J.b_kind ? J.B_hi : J.B;
+    const int ra = (wv >> 1) * 32 + (lane & 31), rb = (wv & 1) * 32 + (lane & 31);
+    const int kh = (lane >> 5) * 8;
+    const float* pd = (const float*)(ADATA ? J.A : J.B);
+    const __bf16* ch = (const __bf16*)(ADATA ? J.B_hi : J.A_hi);
+    const __bf16* cl = (const __bf16*)(ADATA ? J.B_lo : J.A_lo);
+    const int rd = ADATA ? m0 : n0, rc = ADATA ? n0 : m0;
+    const int parts = ADATA ? J.a_parts : J.b_parts;
+    const int64_t pstride = ADATA ? J.a_pstride : J.b_pstride;
+    __bf16(*sD)[2][64][TK + 8] = ADATA ? sA : sB;
+    __bf16(*sC)[2][64][TK + 8] = ADATA ? sB : sA;
     f32x16 acc;
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = 0.0f;
-    const int r = lane & 31, kh = (lane >> 5) * 8;
     const int stages = kper / TK;
-    // stage 0 goes to LDS; stage 1 waits in registers (fa, fb); each stage
-    // then issues the loads of the stage two ahead (na, nb), so two stages'
-    // loads are in flight while one is multiplied
-    Frag<TK> fa = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, kbeg, J.K, J.a_parts, J.a_pstride);
-    Frag<TK> fb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, kbeg, J.K, J.b_parts, J.b_pstride);
-    store_frag<TK>(sA[0], fa);
-    store_frag<TK>(sB[0], fb);
-    if (stages > 1) {
-        fa = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, kbeg + TK, J.K, J.a_parts, J.a_pstride);
-        fb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, kbeg + TK, J.K, J.b_parts, J.b_pstride);
-    }
+    const int klast = kbeg + (stages - 1) * TK;
+    auto kat = [&](int st) { return min(kbeg + st * TK, klast); };
+    Stage<TK> s0, s1;
+    load_stage<TK, SUM>(s0, pd, rd, ch, cl, rc, kbeg, J.K, parts, pstride);
+    load_stage<TK, SUM>(s1, pd, rd, ch, cl, rc, kat(1), J.K, parts, pstride);
+    put_stage<TK>(sD[0], sC[0], s0);
     __syncthreads();
-    for (int st = 0; st < stages; st++) {
-        const int cur = st & 1;
-        const bool more = st + 1 < stages;
-        Frag<TK> na, nb;
-        if (st + 2 < stages) {
-            na = load_frag<TK>(J.a_kind, pa, J.A_lo, m0, kbeg + (st + 2) * TK, J.K, J.a_parts,
-                               J.a_pstride);
-            nb = load_frag<TK>(J.b_kind, pb, J.B_lo, n0, kbeg + (st + 2) * TK, J.K, J.b_parts,
-                               J.b_pstride);
-        }
-#pragma unroll
-        for (int kk = 0; kk < TK; kk += 16) {
-            const bf16x8 ah = *(const bf16x8*)&sA[cur][0][wm * 32 + r][kk + kh];
-            const bf16x8 al = *(const bf16x8*)&sA[cur][1][wm * 32 + r][kk + kh];
-            const bf16x8 bh = *(const bf16x8*)&sB[cur][0][wn * 32 + r][kk + kh];
-            const bf16x8 bl = *(const bf16x8*)&sB[cur][1][wn * 32 + r][kk + kh];
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
-        }
-        if (more) {
-            // the other buffer was last read in stage st - 1, before the
-            // barrier that ended it, so it is free to overwrite
-            store_frag<TK>(sA[cur ^ 1], fa);
-            store_frag<TK>(sB[cur ^ 1], fb);
-        }
+    // Every path through the loop body consumes the loads it carries in (a
+    // load left pending across the back edge would make the wait inserted at
+    // the loop head drain every load in flight), and the scheduling barriers
+    // keep each half's loads at its top, ahead of the split arithmetic that
+    // waits for the previous half's loads.
+    for (int st = 0; st + 1 < stages; st += 2) {
+        // buffer 0 holds stage st, s1 carries stage st + 1
+        load_stage<TK, SUM>(s0, pd, rd, ch, cl, rc, kat(st + 2), J.K, parts, pstride);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_stage<TK>(acc, sA[0], sB[0], ra, rb, kh);
+        put_stage<TK>(sD[1], sC[1], s1);
         __syncthreads();
-        fa = na;
-        fb = nb;
+        // buffer 1 holds stage st + 1, s0 carries stage st + 2
+        load_stage<TK, SUM>(s1, pd, rd, ch, cl, rc, kat(st + 3), J.K, parts, pstride);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_stage<TK>(acc, sA[1], sB[1], ra, rb, kh);
+        put_stage<TK>(sD[0], sC[0], s0);
+        __syncthreads();
     }
+    if (stages & 1) mma_stage<TK>(acc, sA[0], sB[0], ra, rb, kh);  // buffer 0 holds the last stage
     // accumulator layout: element q of lane l is (row 8*(q/4) + 4*(l/32) + q%4, col l%32)
     const size_t po = (size_t)ks * (size_t)J.c_pstride;
+    const int wm = wv >> 1, wn = wv & 1;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int row = m0 + wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
@@ -266,9 +300,10 @@ __global__ __launch_bounds__(256) void k_plane_sum(const PlaneSum* sums) {
     if ((int)blockIdx.x >= S.blocks) return;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= S.n) return;
-    float v = S.p[i];
-    for (int q = 1; q < S.planes; q++) v += S.p[i + q * S.n];
-    S.p[i] = v;
+    EP_GLOBAL float* p = gw<float>(S.p);
+    float v = p[i];
+    for (int q = 1; q < S.planes; q++) v += p[i + q * S.n];
+    p[i] = v;
 }
 
 void launch_plane_sum(const PlaneSum* sums, int n_sums, int max_blocks, hipStream_t s) {
@@ -287,13 +322,13 @@ __device__ inline void frag_direct(int kind, const void* p, const void* plo, int
                                    bf16x8& hi, bf16x8& lo) {
     const size_t off = (size_t)row * K + k;
     if (kind == 0) {
-        const float4* src = (const float4*)((const float*)p + off);
-        const float4 a = src[0], b = src[1];
-        const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        const EP_GLOBAL f32x4* src = gl<f32x4>((const float*)p + off);
+        const f32x4 a = src[0], b = src[1];
+        const float v[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
         split8(v, hi, lo);
     } else {
-        hi = *(const bf16x8*)((const __bf16*)p + off);
-        lo = *(const bf16x8*)((const __bf16*)plo + off);
+        hi = *gl<bf16x8>((const __bf16*)p + off);
+        lo = *gl<bf16x8>((const __bf16*)plo + off);
     }
 }
 
@@ -351,17 +386,30 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_sk(const GemmJob* jobs, int n
     }
 }
 
+template <int TK, bool SUM>
+static void launch_mfma(bool adata, const GemmJob* jobs, int n_jobs, int total_tiles,
+                        const DesignDev* dd, hipStream_t s) {
+    if (adata)
+        hipLaunchKernelGGL((k_gemm_mfma<TK, true, SUM>), dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+    else
+        hipLaunchKernelGGL((k_gemm_mfma<TK, false, SUM>), dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+}
+
 void launch_gemm_pass(const GemmJob* jobs, int n_jobs, int total_tiles, const DesignDev* dd,
-                      int prec, int engine, hipStream_t s) {
+                      int prec, int engine, int pass, hipStream_t s) {
     if (n_jobs <= 0 || total_tiles <= 0) return;
+    // the data operand is A in F1 and I1 (passes 0, 2), B in F2 and I2
+    const bool adata = (pass & 1) == 0;
     if (prec == EP_PREC_FP64)
         hipLaunchKernelGGL(k_gemm_fma<double>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
     else if (engine == 0)
-        hipLaunchKernelGGL(k_gemm_mfma<32>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+        launch_mfma<32, false>(adata, jobs, n_jobs, total_tiles, dd, s);
     else if (engine == 3)
-        hipLaunchKernelGGL(k_gemm_mfma<64>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
-    else if (engine == 4)  // K-sliced jobs
-        hipLaunchKernelGGL(k_gemm_mfma<32>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+        launch_mfma<64, false>(adata, jobs, n_jobs, total_tiles, dd, s);
+    else if (engine == 4 && pass > 0)  // K-sliced jobs reading split planes
+        launch_mfma<32, true>(adata, jobs, n_jobs, total_tiles, dd, s);
+    else if (engine == 4)
+        launch_mfma<32, false>(adata, jobs, n_jobs, total_tiles, dd, s);
     else if (engine == 2)
         hipLaunchKernelGGL(k_gemm_mfma_sk, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
     else

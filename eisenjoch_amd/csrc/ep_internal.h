@@ -196,9 +196,10 @@ void launch_wa_reduce(const BatchDev& b, int prec, int total_cell_blocks, double
                       hipStream_t s);
 
 // ---- transforms (ep_gemm.hip) ----
-// Runs one pass (all jobs) of the spectral solve; engine 0 = MFMA, 1 = FMA.
+// Runs one pass (0 F1, 1 F2, 2 I1, 3 I2; all jobs) of the spectral solve
+// on one engine (0 MFMA, 1 FMA, 2 MFMA split-K, 3 MFMA K stages of 64, 4 K slices).
 void launch_plane_sum(const PlaneSum* sums, int n_sums, int max_blocks, hipStream_t s);
 void launch_gemm_pass(const GemmJob* jobs_dev, int n_jobs, int total_tiles,
-                      const DesignDev* dd, int prec, int engine, hipStream_t s);
+                      const DesignDev* dd, int prec, int engine, int pass, hipStream_t s);
 
 }  // namespace ep

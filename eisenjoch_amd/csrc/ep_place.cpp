@@ -285,7 +285,7 @@ int batch_reset(Ctx* c, Batch& b, const ep_config* cfg) {
 static void enqueue_gemm(Ctx* c, Batch& b, int p) {
     for (int e = 0; e < kEngines; e++)
         if (b.njobs2[p][e] > 0)
-            launch_gemm_pass(b.jobs2[p][e], b.njobs2[p][e], b.tiles2[p][e], b.d_dev, b.prec, e, c->stream);
+            launch_gemm_pass(b.jobs2[p][e], b.njobs2[p][e], b.tiles2[p][e], b.d_dev, b.prec, e, p, c->stream);
     if (p == 3) launch_plane_sum(b.sums, b.n_sums, b.sum_blocks, c->stream);  // field = sum of planes
 }
 
