@@ -43,14 +43,6 @@ __device__ inline EP_GLOBAL T* gw(void* p) {
     return (EP_GLOBAL T*)p;
 }
 
-__device__ inline int find_job(const GemmJob* jobs, int nj, int blk) {
-    int lo = 0, hi = nj;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (jobs[mid].tile_start <= blk) lo = mid; else hi = mid;
-    }
-    return lo;
-}
 
 // Store one output element; kEpiPoisson applies the spectral scaling and
 // the Poisson solve of density.rs:113-153 to coefficient (ky = row, kx = col).
@@ -89,13 +81,13 @@ __device__ inline void epi_store(const GemmJob& J, const DesignDev& D, int row, 
 // --------------------------------------------------------------- FMA -----
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_fma(const GemmJob* jobs, int nj, const DesignDev* dd) {
+__global__ __launch_bounds__(256) void k_gemm_fma(const GemmJob* jobs, const int32_t* bjob, const DesignDev* dd) {
     __shared__ T sA[16][64 + 1];
     __shared__ T sB[16][64 + 1];
-    const int j = find_job(jobs, nj, blockIdx.x);
+    const int j = gl<int32_t>(bjob)[blockIdx.x];
     const GemmJob& J = jobs[j];
+    if (J.st->done) return;
     const DesignDev& D = dd[J.design];
-    if (D.st->done) return;
     const int t = blockIdx.x - J.tile_start;
     const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
     const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
@@ -217,23 +209,24 @@ __device__ inline void mma_stage(f32x16& acc, __bf16 (*sa)[64][TK + 8], __bf16 (
 }
 
 // 64 x 64 output tile per block, four waves each owning a 32 x 32 quadrant,
-// K in stages of TK through two LDS buffers (rows padded by 8 BF16).  Two
+// K in stages of TK through NBUF LDS buffers (rows padded by 8 BF16).  Two
 // register sets alternate (the loop is unrolled by two, so no register copy
 // waits on a load in flight): while a stage is multiplied, the loads of the
 // next two stages are outstanding.  Past the last stage the loads repeat the
-// last stage (in bounds, unused).  TK = 64 halves the chain of stages at
-// twice the LDS (half the resident blocks).  ADATA: the data operand is A
-// (F1, I1), else B (F2, I2).
+// last stage (in bounds, unused).  TK = 64 with one buffer halves the chain
+// of stages in the LDS of TK = 32 with two (40 KB: four resident blocks per
+// CU).  ADATA: the data operand is A (F1, I1), else B (F2, I2).
 // Requires M, N multiples of 64 and K a multiple of TK (checked on the host).
-template <int TK, bool ADATA, bool SUM>
-__global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, const DesignDev* dd) {
-    __shared__ __bf16 sA[2][2][64][TK + 8];  // [buffer][hi/lo][row][k]
-    __shared__ __bf16 sB[2][2][64][TK + 8];
+template <int TK, int NBUF, bool ADATA, bool SUM>
+__global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, const int32_t* bjob, const DesignDev* dd) {
+    __shared__ __bf16 sA[NBUF][2][64][TK + 8];  // [buffer][hi/lo][row][k]
+    __shared__ __bf16 sB[NBUF][2][64][TK + 8];
+    constexpr int b1 = NBUF - 1;  // the second buffer (the first again when single)
     const int tb = blockIdx.x;
-    const int j = find_job(jobs, nj, tb);
+    const int j = gl<int32_t>(bjob)[tb];
     const GemmJob& J = jobs[j];
+    if (J.st->done) return;
     const DesignDev& D = dd[J.design];
-    if (D.st->done) return;
     // block -> (K slice, tile): consecutive blocks take different tiles
     const int tiles = ((J.M + 63) / 64) * J.tiles_n;
     const int tt = tb - J.tile_start;
@@ -267,17 +260,21 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, int nj, 
     // the loop head drain every load in flight), and the scheduling barriers
     // keep each half's loads at its top, ahead of the split arithmetic that
     // waits for the previous half's loads.
+    // With a single LDS buffer, a barrier separates each stage's products
+    // from the put that overwrites it.
     for (int st = 0; st + 1 < stages; st += 2) {
         // buffer 0 holds stage st, s1 carries stage st + 1
         load_stage<TK, SUM>(s0, pd, rd, ch, cl, rc, kat(st + 2), J.K, parts, pstride);
         __builtin_amdgcn_sched_barrier(0);
         mma_stage<TK>(acc, sA[0], sB[0], ra, rb, kh);
-        put_stage<TK>(sD[1], sC[1], s1);
+        if (NBUF == 1) __syncthreads();
+        put_stage<TK>(sD[b1], sC[b1], s1);
         __syncthreads();
-        // buffer 1 holds stage st + 1, s0 carries stage st + 2
+        // buffer b1 holds stage st + 1, s0 carries stage st + 2
         load_stage<TK, SUM>(s1, pd, rd, ch, cl, rc, kat(st + 3), J.K, parts, pstride);
         __builtin_amdgcn_sched_barrier(0);
-        mma_stage<TK>(acc, sA[1], sB[1], ra, rb, kh);
+        mma_stage<TK>(acc, sA[b1], sB[b1], ra, rb, kh);
+        if (NBUF == 1) __syncthreads();
         put_stage<TK>(sD[0], sC[0], s0);
         __syncthreads();
     }
@@ -332,12 +329,12 @@ __device__ inline void frag_direct(int kind, const void* p, const void* plo, int
     }
 }
 
-__global__ __launch_bounds__(256) void k_gemm_mfma_sk(const GemmJob* jobs, int nj, const DesignDev* dd) {
+__global__ __launch_bounds__(256) void k_gemm_mfma_sk(const GemmJob* jobs, const int32_t* bjob, const DesignDev* dd) {
     __shared__ float red[4][64][65];
-    const int j = find_job(jobs, nj, blockIdx.x);
+    const int j = gl<int32_t>(bjob)[blockIdx.x];
     const GemmJob& J = jobs[j];
+    if (J.st->done) return;
     const DesignDev& D = dd[J.design];
-    if (D.st->done) return;
     const int t = blockIdx.x - J.tile_start;
     const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -386,34 +383,36 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_sk(const GemmJob* jobs, int n
     }
 }
 
-template <int TK, bool SUM>
-static void launch_mfma(bool adata, const GemmJob* jobs, int n_jobs, int total_tiles,
+template <int TK, int NBUF, bool SUM>
+static void launch_mfma(bool adata, const GemmJob* jobs, const int32_t* bjob, int total_tiles,
                         const DesignDev* dd, hipStream_t s) {
     if (adata)
-        hipLaunchKernelGGL((k_gemm_mfma<TK, true, SUM>), dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+        hipLaunchKernelGGL((k_gemm_mfma<TK, NBUF, true, SUM>), dim3(total_tiles), dim3(256), 0, s, jobs,
+                           bjob, dd);
     else
-        hipLaunchKernelGGL((k_gemm_mfma<TK, false, SUM>), dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+        hipLaunchKernelGGL((k_gemm_mfma<TK, NBUF, false, SUM>), dim3(total_tiles), dim3(256), 0, s, jobs,
+                           bjob, dd);
 }
 
-void launch_gemm_pass(const GemmJob* jobs, int n_jobs, int total_tiles, const DesignDev* dd,
+void launch_gemm_pass(const GemmJob* jobs, const int32_t* bjob, int total_tiles, const DesignDev* dd,
                       int prec, int engine, int pass, hipStream_t s) {
-    if (n_jobs <= 0 || total_tiles <= 0) return;
+    if (total_tiles <= 0) return;
     // the data operand is A in F1 and I1 (passes 0, 2), B in F2 and I2
     const bool adata = (pass & 1) == 0;
     if (prec == EP_PREC_FP64)
-        hipLaunchKernelGGL(k_gemm_fma<double>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+        hipLaunchKernelGGL(k_gemm_fma<double>, dim3(total_tiles), dim3(256), 0, s, jobs, bjob, dd);
     else if (engine == 0)
-        launch_mfma<32, false>(adata, jobs, n_jobs, total_tiles, dd, s);
-    else if (engine == 3)
-        launch_mfma<64, false>(adata, jobs, n_jobs, total_tiles, dd, s);
+        launch_mfma<32, 2, false>(adata, jobs, bjob, total_tiles, dd, s);
+    else if (engine == 3)  // K stages of 64 through one LDS buffer (the LDS of engine 0)
+        launch_mfma<64, 1, false>(adata, jobs, bjob, total_tiles, dd, s);
     else if (engine == 4 && pass > 0)  // K-sliced jobs reading split planes
-        launch_mfma<32, true>(adata, jobs, n_jobs, total_tiles, dd, s);
+        launch_mfma<32, 2, true>(adata, jobs, bjob, total_tiles, dd, s);
     else if (engine == 4)
-        launch_mfma<32, false>(adata, jobs, n_jobs, total_tiles, dd, s);
+        launch_mfma<32, 2, false>(adata, jobs, bjob, total_tiles, dd, s);
     else if (engine == 2)
-        hipLaunchKernelGGL(k_gemm_mfma_sk, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+        hipLaunchKernelGGL(k_gemm_mfma_sk, dim3(total_tiles), dim3(256), 0, s, jobs, bjob, dd);
     else
-        hipLaunchKernelGGL(k_gemm_fma<float>, dim3(total_tiles), dim3(256), 0, s, jobs, n_jobs, dd);
+        hipLaunchKernelGGL(k_gemm_fma<float>, dim3(total_tiles), dim3(256), 0, s, jobs, bjob, dd);
 }
 
 }  // namespace ep

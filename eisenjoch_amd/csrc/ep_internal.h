@@ -148,6 +148,7 @@ struct GemmJob {
     int32_t ksplit;            // K slices (one block per tile and slice), MFMA engines 0/3
     int32_t a_parts, b_parts;  // partial planes summed, in order, when loading a data operand
     int32_t pad;
+    const DevState* st;        // the design's state (a block of a done design returns at once)
     int64_t c_pstride;         // elements between the output's partial planes (C and C2)
     int64_t a_pstride, b_pstride;  // elements between a data operand's planes
 };
@@ -199,7 +200,8 @@ void launch_wa_reduce(const BatchDev& b, int prec, int total_cell_blocks, double
 // Runs one pass (0 F1, 1 F2, 2 I1, 3 I2; all jobs) of the spectral solve
 // on one engine (0 MFMA, 1 FMA, 2 MFMA split-K, 3 MFMA K stages of 64, 4 K slices).
 void launch_plane_sum(const PlaneSum* sums, int n_sums, int max_blocks, hipStream_t s);
-void launch_gemm_pass(const GemmJob* jobs_dev, int n_jobs, int total_tiles,
+// bjob: the job of each block (total_tiles entries).
+void launch_gemm_pass(const GemmJob* jobs_dev, const int32_t* bjob, int total_tiles,
                       const DesignDev* dd, int prec, int engine, int pass, hipStream_t s);
 
 }  // namespace ep

@@ -113,6 +113,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             GemmJob J;
             memset(&J, 0, sizeof(J));
             J.M = M; J.N = N; J.K = K; J.epi = epi; J.C = C; J.C2 = C2; J.design = i;
+            J.st = D.dev.st;
             J.tiles_n = (N + 63) / 64;
             J.ksplit = 1;
             J.a_parts = J.b_parts = 1;
@@ -168,16 +169,20 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     b.tile_blocks = pt[nd];
     // device image: descriptors, prefixes, jobs
     size_t bytes = align_up(sizeof(DesignDev) * nd) + 6 * align_up(sizeof(int32_t) * (nd + 1)) + align_up(4);
+    std::vector<int32_t> bjob[4][kEngines];
     for (int p = 0; p < 4; p++)
         for (int e = 0; e < kEngines; e++) {
             int t0 = 0;
-            for (auto& J : jobs[p][e]) {
+            for (size_t q = 0; q < jobs[p][e].size(); q++) {
+                GemmJob& J = jobs[p][e][q];
                 J.tile_start = t0;
                 t0 += ((J.M + 63) / 64) * J.tiles_n * J.ksplit;
+                bjob[p][e].resize(t0, (int32_t)q);
             }
             b.tiles2[p][e] = t0;
             b.njobs2[p][e] = (int)jobs[p][e].size();
             bytes += align_up(sizeof(GemmJob) * std::max<size_t>(1, jobs[p][e].size()));
+            bytes += align_up(sizeof(int32_t) * std::max<size_t>(1, bjob[p][e].size()));
         }
     bytes += align_up(sizeof(PlaneSum) * std::max<size_t>(1, sums.size()));
     HIPCHK(c, hipMalloc(&b.mem, bytes));
@@ -206,6 +211,8 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             memset(&dummy, 0, sizeof(dummy));
             if (jobs[p][e].empty()) b.jobs2[p][e] = (GemmJob*)put(&dummy, sizeof(GemmJob));
             else b.jobs2[p][e] = (GemmJob*)put(jobs[p][e].data(), sizeof(GemmJob) * jobs[p][e].size());
+            if (bjob[p][e].empty()) b.bjob2[p][e] = (int32_t*)put(&zero, sizeof(int32_t));
+            else b.bjob2[p][e] = (int32_t*)put(bjob[p][e].data(), sizeof(int32_t) * bjob[p][e].size());
         }
     {
         PlaneSum none;
@@ -285,7 +292,8 @@ int batch_reset(Ctx* c, Batch& b, const ep_config* cfg) {
 static void enqueue_gemm(Ctx* c, Batch& b, int p) {
     for (int e = 0; e < kEngines; e++)
         if (b.njobs2[p][e] > 0)
-            launch_gemm_pass(b.jobs2[p][e], b.njobs2[p][e], b.tiles2[p][e], b.d_dev, b.prec, e, p, c->stream);
+            launch_gemm_pass(b.jobs2[p][e], b.bjob2[p][e], b.tiles2[p][e], b.d_dev, b.prec, e, p,
+                             c->stream);
     if (p == 3) launch_plane_sum(b.sums, b.n_sums, b.sum_blocks, c->stream);  // field = sum of planes
 }
 

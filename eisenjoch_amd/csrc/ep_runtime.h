@@ -64,6 +64,7 @@ struct Batch {
     int njobs2[4][kEngines] = {};
     int tiles2[4][kEngines] = {};
     GemmJob* jobs2[4][kEngines] = {};
+    int32_t* bjob2[4][kEngines] = {};  // job of each block of a pass (one load at block start)
     PlaneSum* sums = nullptr;        // split final passes (device), summed after pass 3
     int n_sums = 0, sum_blocks = 0;
     void* mem = nullptr;        // descriptors, prefixes, jobs

@@ -204,7 +204,8 @@ int ep_profile_iteration(ep_ctx* ctx, const uint32_t* ids, uint32_t n, const ep_
 /* Autotuner (no counterpart in the reference): for each distinct bin shape
  * among the FAST designs `ids`, largest first, times every eligible
  * transform engine (engine 0: MFMA with LDS staging in K stages of 32;
- * 1: FP32 FMA; 2: MFMA split-K inside a block; 3: MFMA with K stages of 64;
+ * 1: FP32 FMA; 2: MFMA split-K inside a block; 3: MFMA with K stages of 64
+ * through one LDS buffer;
  * 4: engine 0 with products of K >= 512 split across blocks into K slices
  * of 128) on the four
  * transform passes of the whole batch `ids`, `reps` times, with the other
