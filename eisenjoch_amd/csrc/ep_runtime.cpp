@@ -144,6 +144,7 @@ static void build_layout(const ep_design* d, NetLayout& L) {
     std::vector<std::vector<std::pair<uint32_t, int>>> nets(kMaxBuckets);
     L.hf_off.assign(1, 0);
     const uint32_t nm = (uint32_t)d->n_mov;
+    std::vector<uint32_t> med;
     for (int e = 0; e < d->n_nets; e++) {
         uint32_t b0 = d->net_off[e], n = d->net_off[e + 1] - b0;
         if (n < 2) continue;
@@ -155,9 +156,16 @@ static void build_layout(const ep_design* d, NetLayout& L) {
         }
         int b = 0;
         while ((2u << b) < n) b++;
-        uint32_t key = 0xFFFFFFFFu;
+        // key: the median movable pin (a net's slots then sit beside the nets
+        // of most of its cells, not beside one stray pin's)
+        med.clear();
         for (uint32_t k = 0; k < n; k++)
-            if (d->pin_cell[b0 + k] < nm) key = std::min(key, d->pin_cell[b0 + k]);
+            if (d->pin_cell[b0 + k] < nm) med.push_back(d->pin_cell[b0 + k]);
+        uint32_t key = 0xFFFFFFFFu;
+        if (!med.empty()) {
+            std::nth_element(med.begin(), med.begin() + med.size() / 2, med.end());
+            key = med[med.size() / 2];
+        }
         nets[b].push_back({key, e});
     }
     int waves = 0;
