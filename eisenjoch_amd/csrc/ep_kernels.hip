@@ -623,43 +623,61 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
     const int stx = (D.tiles_x + 3) / 4;
     const int sx = t % stx, sy = t / stx;
     const int x0 = sx * 32, y0 = sy * 32;
-    for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) win[k] = Acc(0);
-    __syncthreads();
-    const T* SXY = (const T*)D.sorted_xy;
-    const float* SHS = D.sorted_hs[(D.st->it & 1) ^ 1];
-    // ranges: q = 0 the 4x4 block of tiles (Morton keys 16*m .. 16*m+15),
-    // q = 1..9 the halo tiles (left column and bottom row, corner included)
-    for (int q = 0; q < 10; q++) {
-        uint32_t k0, k1;
+    // the ten key ranges: q = 0 the 4x4 block of tiles (Morton keys 16*m ..
+    // 16*m+15), q = 1..9 the halo tiles (left column and bottom row, corner
+    // included).  Their bounds are loaded at once, by ten threads, and the
+    // cells of all ten are then walked as one index space, so every cell
+    // load of the block is issued in the same pass.
+    __shared__ uint32_t rk0[10], rlen[10];
+    if (threadIdx.x < 10) {
+        const int q = threadIdx.x;
+        uint32_t k0 = 0, k1 = 0;
         if (q == 0) {
             const uint32_t m = morton2((uint32_t)sx, (uint32_t)sy) * 16u;
-            if (m >= (uint32_t)D.n_keys) continue;
-            k0 = D.tstart[m];
-            k1 = D.tstart[min(m + 16u, (uint32_t)D.n_keys)];
+            if (m < (uint32_t)D.n_keys) {
+                k0 = D.tstart[m];
+                k1 = D.tstart[min(m + 16u, (uint32_t)D.n_keys)];
+            }
         } else {
             int hx, hy;
             if (q <= 5) { hx = sx * 4 - 1; hy = sy * 4 - 1 + (q - 1); }  // left column, from the corner
             else { hx = sx * 4 + (q - 6); hy = sy * 4 - 1; }            // bottom row
-            if (hx < 0 || hy < 0) continue;
-            const uint32_t key = morton2((uint32_t)hx, (uint32_t)hy);
-            if (key >= (uint32_t)D.n_keys) continue;
-            k0 = D.tstart[key];
-            k1 = D.tstart[key + 1];
-        }
-        for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-            const BinBox r = bin_box(D, (double)SXY[2 * (size_t)k], (double)SXY[2 * (size_t)k + 1],
-                                     (double)SHS[k]);
-            const int bx0 = max(r.bx0, x0), bx1 = min(r.bx1, x0 + 32);
-            const int by0 = max(r.by0, y0), by1 = min(r.by1, y0 + 32);
-            for (int by = by0; by < by1; by++)
-                for (int bx = bx0; bx < bx1; bx++) {
-                    const double w = ov_area(r, bx, by);
-                    Acc v;
-                    if constexpr (FIX) v = (Acc)__double2ll_rn(w * kFixScale);
-                    else v = w;
-                    atomicAdd(&win[(by - y0) * 32 + (bx - x0)], v);
+            if (hx >= 0 && hy >= 0) {
+                const uint32_t key = morton2((uint32_t)hx, (uint32_t)hy);
+                if (key < (uint32_t)D.n_keys) {
+                    k0 = D.tstart[key];
+                    k1 = D.tstart[key + 1];
                 }
+            }
         }
+        rk0[q] = k0;
+        rlen[q] = k1 - k0;
+    }
+    for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) win[k] = Acc(0);
+    __syncthreads();
+    uint32_t pre[11];  // start of range q in the joint index space
+    pre[0] = 0;
+#pragma unroll
+    for (int q = 0; q < 10; q++) pre[q + 1] = pre[q] + rlen[q];
+    const T* SXY = (const T*)D.sorted_xy;
+    const float* SHS = D.sorted_hs[(D.st->it & 1) ^ 1];
+    for (uint32_t j = threadIdx.x; j < pre[10]; j += blockDim.x) {
+        uint32_t k = 0;
+#pragma unroll
+        for (int q = 0; q < 10; q++)  // the last range starting at or before j holds it
+            if (j >= pre[q]) k = rk0[q] + (j - pre[q]);
+        const BinBox r = bin_box(D, (double)SXY[2 * (size_t)k], (double)SXY[2 * (size_t)k + 1],
+                                 (double)SHS[k]);
+        const int bx0 = max(r.bx0, x0), bx1 = min(r.bx1, x0 + 32);
+        const int by0 = max(r.by0, y0), by1 = min(r.by1, y0 + 32);
+        for (int by = by0; by < by1; by++)
+            for (int bx = bx0; bx < bx1; bx++) {
+                const double w = ov_area(r, bx, by);
+                Acc v;
+                if constexpr (FIX) v = (Acc)__double2ll_rn(w * kFixScale);
+                else v = w;
+                atomicAdd(&win[(by - y0) * 32 + (bx - x0)], v);
+            }
     }
     __syncthreads();
     double p[kBinParts] = {0.0, 0.0};
