@@ -174,6 +174,14 @@ struct BatchDev {
     const int32_t* pfx_hf;     // one block per high-degree net
     const int32_t* pfx_scat;   // atomic scatter blocks (kScatCells cells each)
     const int32_t* pfx_tile;   // pull scatter blocks (one per 8x8-bin tile)
+    // design of each block of the same six grids (one load at block start;
+    // null in a one-design view, where the design is 0)
+    const int32_t* blk_cell;
+    const int32_t* blk_bin;
+    const int32_t* blk_wa;
+    const int32_t* blk_hf;
+    const int32_t* blk_scat;
+    const int32_t* blk_tile;
 };
 
 // ---- launchers (ep_kernels.hip) ----

@@ -58,6 +58,12 @@ __device__ inline int find_blk(const int32_t* pfx, int nd, int blk) {
     return lo;
 }
 
+// The design of this block: from the block's table entry, or (one-design
+// views, no table) by search.
+__device__ inline int design_of(const int32_t* tab, const int32_t* pfx, int nd, int blk) {
+    return tab ? tab[blk] : find_blk(pfx, nd, blk);
+}
+
 template <typename T>
 __device__ inline T shfl_xor(T v, int o) { return __shfl_xor(v, o, kWave); }
 
@@ -142,7 +148,7 @@ __device__ inline void store_grad(const DesignDev& D, uint32_t slot, T gx, T gy)
 // solver positions; else the WA gradient of the look-ahead positions.
 template <typename T, bool FIX, bool HPWL>
 __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
-    const int d = find_blk(b.pfx_wa, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_wa, b.pfx_wa, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     const DevState* st = D.st;
     if (HPWL) {
@@ -225,7 +231,7 @@ template <typename T, bool FIX, bool HPWL>
 __global__ __launch_bounds__(kHfThreads) void k_wa_hf(BatchDev b, int final_pass) {
     __shared__ double sh[kHfWaves * 8];
     __shared__ double tot[8];
-    const int d = find_blk(b.pfx_hf, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_hf, b.pfx_hf, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     const DevState* st = D.st;
     if (HPWL) {
@@ -365,7 +371,7 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
     typedef typename std::conditional<FIX, unsigned long long, double>::type Acc;
     __shared__ Acc win[kScatBins];
     __shared__ int box[4];
-    const int d = find_blk(b.pfx_scat, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_scat, b.pfx_scat, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
     const int c0 = (blockIdx.x - b.pfx_scat[d]) * kScatCells;
@@ -424,7 +430,7 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_dens_final(BatchDev b) {
     __shared__ double sh[4 * kBinParts];
-    const int d = find_blk(b.pfx_bin, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_bin, b.pfx_bin, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
     const int blk = blockIdx.x - b.pfx_bin[d];
@@ -499,7 +505,7 @@ __device__ inline uint32_t wave_agg_add(uint32_t* ctr, uint32_t key, bool valid)
 // contiguous arrays and places each cell at tstart[key] + offset.
 template <typename T>
 __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
-    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
     const int k = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
@@ -585,7 +591,7 @@ __global__ __launch_bounds__(256) void k_tile_scan(BatchDev b) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_tile_fill(BatchDev b) {
-    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
     const int k = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
@@ -616,7 +622,7 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
     typedef typename std::conditional<FIX, unsigned long long, double>::type Acc;
     __shared__ Acc win[32 * 32];
     __shared__ double sh[4 * kBinParts];
-    const int d = find_blk(b.pfx_tile, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_tile, b.pfx_tile, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
     const int t = blockIdx.x - b.pfx_tile[d];
@@ -760,7 +766,7 @@ __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy)
 // kernels, beside the density chain.
 template <typename T>
 __global__ __launch_bounds__(256) void k_wl_cell(BatchDev b) {
-    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
     const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
@@ -775,7 +781,7 @@ __global__ __launch_bounds__(256) void k_wl_cell(BatchDev b) {
 template <typename T, bool FIX, int MODE>
 __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, double* ey_out) {
     __shared__ double sh[4 * 4];
-    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     const DevState* st = D.st;
     if (st->done) return;
@@ -785,6 +791,20 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
     const int it = st->it, par = it & 1;
     double part[4] = {0, 0, 0, 0};
     if (i < D.n_mov) {
+        // the cell's own operands of the combine step are loaded first, beside
+        // its position, so their latency overlaps the field loads (dpos and the
+        // previous gradient hold stale values at it = 0 and are then unused)
+        double2 wl = make_double2(0.0, 0.0);
+        double pw = 0.0, dpv[2] = {0.0, 0.0}, gpv[2] = {0.0, 0.0};
+        if (MODE == 0) {
+            wl = ((const double2*)D.gwl)[i];
+            pw = (double)((const T*)D.pinw)[i];
+#pragma unroll
+            for (int ax = 0; ax < 2; ax++) {
+                dpv[ax] = (double)((const T*)D.dpos[ax])[i];
+                gpv[ax] = (double)((const T*)D.g[par ^ 1][ax])[i];
+            }
+        }
         double gden[2] = {0, 0};
         if (MODE != 1) {
             const typename Vec2<T>::type l = ((const typename Vec2<T>::type*)D.look)[i];
@@ -844,7 +864,6 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
             ey_out[i] = gden[1];
         } else if (MODE == 0 && !st->lam_init) {
             // penalty not yet initialised: park the density gradient, norms only
-            const double2 wl = ((const double2*)D.gwl)[i];
             const double w0 = wl.x, w1 = wl.y;
             ((T*)D.g[par][0])[i] = (T)gden[0];
             ((T*)D.g[par][1])[i] = (T)gden[1];
@@ -852,9 +871,12 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
             part[1] = gden[0] * gden[0] + gden[1] * gden[1];
         } else {
             const double lam = MODE == 0 ? sched(D.cfg, st->lam) : st->lam;
-            double pc = ((const T*)D.pinw)[i] + lam;  // mod.rs:152-156
+            if (MODE != 0) {  // the fix pass reads its operands here
+                wl = ((const double2*)D.gwl)[i];
+                pw = (double)((const T*)D.pinw)[i];
+            }
+            double pc = pw + lam;  // mod.rs:152-156
             pc = pc < 1.0 ? 1.0 : pc;
-            const double2 wl = ((const double2*)D.gwl)[i];
             const double gwa[2] = {wl.x, wl.y};
 #pragma unroll
             for (int ax = 0; ax < 2; ax++) {
@@ -862,8 +884,8 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
                 T g = (T)((gw + lam * gden[ax]) / pc);  // mod.rs:146-149
                 ((T*)D.g[par][ax])[i] = g;
                 if (it > 0) {
-                    double dx = (double)((const T*)D.dpos[ax])[i];
-                    double dg = (double)g - (double)((const T*)D.g[par ^ 1][ax])[i];
+                    double dx = MODE == 0 ? dpv[ax] : (double)((const T*)D.dpos[ax])[i];
+                    double dg = (double)g - (MODE == 0 ? gpv[ax] : (double)((const T*)D.g[par ^ 1][ax])[i]);
                     part[2 * ax + 0] = dx * dx;
                     part[2 * ax + 1] = dx * dg;
                 }
@@ -1046,7 +1068,7 @@ __global__ __launch_bounds__(256) void k_ctrl(BatchDev b, int which) {
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_update(BatchDev b) {
     __shared__ double sh[4 * 2];
-    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     const DevState* st = D.st;
     if (!st->run_update) return;
@@ -1094,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_update(BatchDev b) {
 // nesterov.rs:114-118; first look-ahead has zero momentum, nesterov.rs:75-82).
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_reset(BatchDev b) {
-    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
@@ -1118,7 +1140,7 @@ __global__ __launch_bounds__(256) void k_reset(BatchDev b) {
 // Evaluation helper: the WA gradient per cell (the gather's slot reduction).
 template <typename T>
 __global__ __launch_bounds__(256) void k_wa_reduce(BatchDev b, double* gx, double* gy) {
-    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
@@ -1130,7 +1152,7 @@ __global__ __launch_bounds__(256) void k_wa_reduce(BatchDev b, double* gx, doubl
 
 // Evaluation helper: integer bin ranges per cell.
 __global__ __launch_bounds__(256) void k_bin_ranges(BatchDev b, int32_t* out) {
-    const int d = find_blk(b.pfx_cell, b.nd, blockIdx.x);
+    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
     const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;

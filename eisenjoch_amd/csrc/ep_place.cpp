@@ -185,6 +185,16 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             bytes += align_up(sizeof(int32_t) * std::max<size_t>(1, bjob[p][e].size()));
         }
     bytes += align_up(sizeof(PlaneSum) * std::max<size_t>(1, sums.size()));
+    // block -> design tables of the six grids
+    const std::vector<int32_t>* pfxs[6] = {&pc, &pb, &pw, &ph, &ps, &pt};
+    std::vector<int32_t> blk[6];
+    for (int g = 0; g < 6; g++) {
+        const std::vector<int32_t>& P = *pfxs[g];
+        blk[g].resize(std::max(1, P[nd]), 0);
+        for (int i = 0; i < nd; i++)
+            std::fill(blk[g].begin() + P[i], blk[g].begin() + P[i + 1], i);
+        bytes += align_up(sizeof(int32_t) * blk[g].size());
+    }
     HIPCHK(c, hipMalloc(&b.mem, bytes));
     std::vector<char> img(bytes, 0);
     size_t off = 0;
@@ -203,6 +213,9 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     b.bd.pfx_hf = (const int32_t*)put(ph.data(), 4 * (nd + 1));
     b.bd.pfx_scat = (const int32_t*)put(ps.data(), 4 * (nd + 1));
     b.bd.pfx_tile = (const int32_t*)put(pt.data(), 4 * (nd + 1));
+    const int32_t** blkp[6] = {&b.bd.blk_cell, &b.bd.blk_bin, &b.bd.blk_wa, &b.bd.blk_hf, &b.bd.blk_scat,
+                               &b.bd.blk_tile};
+    for (int g = 0; g < 6; g++) *blkp[g] = (const int32_t*)put(blk[g].data(), sizeof(int32_t) * blk[g].size());
     const int32_t zero = 0;
     b.dzero = (const int32_t*)put(&zero, 4);
     for (int p = 0; p < 4; p++)
@@ -276,6 +289,7 @@ int batch_reset_some(Ctx* c, Batch& b, const std::vector<int>& idx) {
             one.d = b.d_dev + i;
             one.nd = 1;
             one.pfx_cell = b.dzero;
+            one.blk_cell = nullptr;
             launch_reset(one, b.prec, b.ds[i]->dev.n_cell_blocks, c->stream);
         }
     }

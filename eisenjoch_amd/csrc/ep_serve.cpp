@@ -63,7 +63,8 @@ static BatchDev single_view(const Batch& b, int i) {
     BatchDev one = b.bd;
     one.d = b.d_dev + i;
     one.nd = 1;
-    one.pfx_cell = one.pfx_bin = one.pfx_wa = one.pfx_hf = one.pfx_scat = b.dzero;
+    one.pfx_cell = one.pfx_bin = one.pfx_wa = one.pfx_hf = one.pfx_scat = one.pfx_tile = b.dzero;
+    one.blk_cell = one.blk_bin = one.blk_wa = one.blk_hf = one.blk_scat = one.blk_tile = nullptr;
     return one;
 }
 
