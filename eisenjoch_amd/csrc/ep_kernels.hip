@@ -598,14 +598,15 @@ __global__ __launch_bounds__(256) void k_tile_fill(BatchDev b) {
     const bool valid = k < D.n_mov;
     const int par = D.st->it & 1;
     const uint32_t i = valid ? D.perm[par][k] : 0u;
+    // the staged copy is loaded beside the key, not after the position
+    typedef typename Vec2<T>::type V;
+    const V xy = valid ? ((const V*)D.stage_xy)[k] : V{};
+    const float hs = valid ? D.stage_hs[k] : 0.0f;
     const uint32_t pos = valid ? D.tstart[D.keys[k]] + D.tofs[k] : 0u;
     if (valid && pos < (uint32_t)D.n_mov) {
-        const T* gxy = (const T*)D.stage_xy;
         D.perm[par ^ 1][pos] = i;
-        T* sxy = (T*)D.sorted_xy;
-        sxy[2 * (size_t)pos] = gxy[2 * (size_t)k];
-        sxy[2 * (size_t)pos + 1] = gxy[2 * (size_t)k + 1];
-        D.sorted_hs[par ^ 1][pos] = D.stage_hs[k];
+        ((V*)D.sorted_xy)[pos] = xy;
+        D.sorted_hs[par ^ 1][pos] = hs;
     }
 }
 
@@ -710,9 +711,9 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
 // WA gradient of cell i: its pin slots (nets of <= 64 pins) summed in
 // ascending slot order, then its pins on high-fanout nets, in ascending net
 // order, each formed from the net's statistics exactly as a slot would hold
-// it.  The slot indices and slot values of four pins are loaded before any
-// is added, so four dependent load pairs are in flight at once; the
-// additions keep the sequential order.
+// it.  The slot indices and slot values of up to eight pins (then four) are
+// loaded before any is added, so their dependent load pairs are in flight at
+// once; the additions keep the sequential order.
 template <typename T>
 __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy) {
     typedef typename Vec2<T>::type V;
@@ -720,6 +721,25 @@ __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy)
     double sx = 0.0, sy = 0.0;
     uint32_t k = D.cs_off[i];
     const uint32_t e = D.cs_off[i + 1];
+    if (e > k) {
+        // the first eight pins (most cells have no more) in one round: all
+        // their slot indices, then all their slot values, are loaded at once
+        // (indices past the cell's last pin repeat it and are not added)
+        const uint32_t n1 = min(e - k, 8u);
+        uint32_t s8[8];
+        V v8[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) s8[q] = D.cs_idx[k + min((uint32_t)q, n1 - 1)];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v8[q] = pg[s8[q]];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if ((uint32_t)q < n1) {
+                sx += (double)v8[q].x;
+                sy += (double)v8[q].y;
+            }
+        k += n1;
+    }
     for (; k + 4 <= e; k += 4) {
         const uint32_t s0 = D.cs_idx[k], s1 = D.cs_idx[k + 1];
         const uint32_t s2 = D.cs_idx[k + 2], s3 = D.cs_idx[k + 3];
@@ -1078,6 +1098,16 @@ __global__ __launch_bounds__(256) void k_update(BatchDev b) {
     double sq[2] = {0, 0};
     if (i < D.n_mov) {
         const T m = (T)st->m, mn = (T)st->m_next;
+        // both axes' operands are loaded before either axis stores (the
+        // stores could alias the other axis's loads as far as the compiler
+        // knows, which would order each axis's loads after the other's stores)
+        T xa[2], dxa[2], ga[2];
+#pragma unroll
+        for (int ax = 0; ax < 2; ax++) {
+            xa[ax] = ((const T*)D.pos[ax])[i];
+            dxa[ax] = ((const T*)D.dpos[ax])[i];
+            ga[ax] = ((const T*)D.g[par][ax])[i];
+        }
         T lk[2];
 #pragma unroll
         for (int ax = 0; ax < 2; ax++) {
@@ -1085,8 +1115,8 @@ __global__ __launch_bounds__(256) void k_update(BatchDev b) {
             const T s = (T)st->s[ax];
             T* P = (T*)D.pos[ax];
             T* DP = (T*)D.dpos[ax];
-            const T x = P[i], dx = DP[i];
-            const T g = ((const T*)D.g[par][ax])[i];
+            const T x = xa[ax], dx = dxa[ax];
+            const T g = ga[ax];
             const T yv = x + m * dx;  // nesterov.rs:55: y = x + m (x - x_prev)
             const T xn = yv - s * g;  // nesterov.rs:56
             const double dl = (double)xn - (double)x;
