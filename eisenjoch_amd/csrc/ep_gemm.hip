@@ -25,6 +25,7 @@ namespace ep {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // Operand and output pointers come from job descriptors in memory; as
 // generic pointers they would compile to flat accesses, which count on the
@@ -44,9 +45,28 @@ __device__ inline EP_GLOBAL T* gw(void* p) {
 }
 
 
-// Store one output element; kEpiPoisson applies the spectral scaling and
-// the Poisson solve of density.rs:113-153 to coefficient (ky = row, kx = col).
-// po: element offset of the output plane (K slice) being written.
+// The spectral scaling and Poisson solve of density.rs:113-153 for
+// coefficient v at (ky = row, kx = col), given the spectral factors
+// (w, 2(1 - cos w)) of its column (wx, lx) and row (wy, ly).
+__device__ inline void poisson(const DesignDev& D, int row, int col, double v, double wx, double lx,
+                               double wy, double ly, double& ex, double& ey) {
+    double s = 4.0 / ((double)D.Bx * (double)D.By);
+    if (col == 0) s *= 0.5;
+    if (row == 0) s *= 0.5;
+    const double a = v * s;
+    const double den = lx + ly;
+    ex = 0.0;
+    ey = 0.0;
+    if (!(den < 1e-30)) {
+        const double phi = a / den;
+        ex = phi * wx;
+        ey = phi * wy;
+    }
+}
+
+// Store one output element; kEpiPoisson applies poisson() to coefficient
+// (ky = row, kx = col).  po: element offset of the output plane (K slice)
+// being written.
 template <typename T>
 __device__ inline void epi_store(const GemmJob& J, const DesignDev& D, int row, int col, double v,
                                  size_t po = 0) {
@@ -60,19 +80,9 @@ __device__ inline void epi_store(const GemmJob& J, const DesignDev& D, int row, 
     } else {
         const EP_GLOBAL T* sx = gl<T>(D.spec_x);  // [kx] -> (w, 2(1 - cos w))
         const EP_GLOBAL T* sy = gl<T>(D.spec_y);
-        double s = 4.0 / ((double)D.Bx * (double)D.By);
-        if (col == 0) s *= 0.5;
-        if (row == 0) s *= 0.5;
-        const double a = v * s;
-        const double wx = (double)sx[2 * col], lx = (double)sx[2 * col + 1];
-        const double wy = (double)sy[2 * row], ly = (double)sy[2 * row + 1];
-        const double den = lx + ly;
-        double ex = 0.0, ey = 0.0;
-        if (!(den < 1e-30)) {
-            const double phi = a / den;
-            ex = phi * wx;
-            ey = phi * wy;
-        }
+        double ex, ey;
+        poisson(D, row, col, v, (double)sx[2 * col], (double)sx[2 * col + 1], (double)sy[2 * row],
+                (double)sy[2 * row + 1], ex, ey);
         gw<T>(J.C)[po + (size_t)row * J.N + col] = (T)ex;
         gw<T>(J.C2)[po + (size_t)row * J.N + col] = (T)ey;
     }
@@ -282,10 +292,32 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, const in
     // accumulator layout: element q of lane l is (row 8*(q/4) + 4*(l/32) + q%4, col l%32)
     const size_t po = (size_t)ks * (size_t)J.c_pstride;
     const int wm = wv >> 1, wn = wv & 1;
+    const int col = n0 + wn * 32 + (lane & 31);
+    if (J.epi == kEpiPoisson) {
+        // the spectral factors of the lane's column and sixteen rows are all
+        // loaded before the first store (the stores could alias them as far
+        // as the compiler knows, which would order each load after the
+        // previous element's store)
+        const EP_GLOBAL f32x2* sx = gl<f32x2>(D.spec_x);
+        const EP_GLOBAL f32x2* sy = gl<f32x2>(D.spec_y);
+        const f32x2 fx = sx[col];
+        f32x2 fy[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) fy[q] = sy[m0 + wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int row = m0 + wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+            double ex, ey;
+            poisson(D, row, col, (double)acc[q], (double)fx[0], (double)fx[1], (double)fy[q][0],
+                    (double)fy[q][1], ex, ey);
+            gw<float>(J.C)[po + (size_t)row * J.N + col] = (float)ex;
+            gw<float>(J.C2)[po + (size_t)row * J.N + col] = (float)ey;
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int row = m0 + wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-        const int col = n0 + wn * 32 + (lane & 31);
         epi_store<float>(J, D, row, col, (double)acc[q], po);
     }
 }

@@ -477,23 +477,35 @@ __device__ inline uint32_t tile_key(const DesignDev& D, T x, T y, float hs) {
 }
 
 // One atomic per distinct key of a wave: lanes with equal keys are counted
-// together (the previous order keeps them mostly equal); with want_pos each
-// lane also receives its own slot (base + rank among the lanes of its key).
+// together (the previous order keeps them mostly equal), and each lane
+// receives its own slot (the key's count before the add + its rank among the
+// lanes of its key).
 __device__ inline uint32_t wave_agg_add(uint32_t* ctr, uint32_t key, bool valid) {
     const int lane = threadIdx.x & 63;
     unsigned long long act = __ballot(valid);
-    uint32_t mine = 0;
+    // group the lanes by key (register work only), then every group's leader
+    // adds its group's count in one atomic instruction: one round trip for
+    // the wave however many keys it holds
+    bool leader = false;
+    uint32_t cnt = 0, rank = 0;
+    int lead = 0;
     while (act) {
         const int l = __ffsll((long long)act) - 1;
         const uint32_t lk = __shfl(key, l, kWave);
         const unsigned long long same = __ballot(valid && key == lk) & act;
-        uint32_t base = 0;
-        if (lane == l) base = atomicAdd(&ctr[lk], (uint32_t)__popcll(same));
-        base = __shfl(base, l, kWave);
-        if ((same >> lane) & 1ull) mine = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        if (lane == l) {
+            leader = true;
+            cnt = (uint32_t)__popcll(same);
+        }
+        if ((same >> lane) & 1ull) {
+            lead = l;
+            rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        }
         act &= ~same;
     }
-    return mine;
+    uint32_t base = 0;
+    if (leader) base = atomicAdd(&ctr[key], cnt);
+    return __shfl(base, lead, kWave) + rank;
 }
 
 // Counting sort of the cells by tile key, every iteration: count, scan, fill.
