@@ -94,11 +94,13 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_fma(const GemmJob* jobs, const int32_t* bjob, const DesignDev* dd) {
     __shared__ T sA[16][64 + 1];
     __shared__ T sB[16][64 + 1];
-    const int j = gl<int32_t>(bjob)[blockIdx.x];
+    const int j = gl<int32_t>(bjob)[2 * blockIdx.x];
+    if (j < 0) return;
+    const int tb = gl<int32_t>(bjob)[2 * blockIdx.x + 1];
     const GemmJob& J = jobs[j];
     if (J.st->done) return;
     const DesignDev& D = dd[J.design];
-    const int t = blockIdx.x - J.tile_start;
+    const int t = tb - J.tile_start;
     const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
     const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
     const EP_GLOBAL T* A = gl<T>(J.A);
@@ -232,8 +234,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, const in
     __shared__ __bf16 sA[NBUF][2][64][TK + 8];  // [buffer][hi/lo][row][k]
     __shared__ __bf16 sB[NBUF][2][64][TK + 8];
     constexpr int b1 = NBUF - 1;  // the second buffer (the first again when single)
-    const int tb = blockIdx.x;
-    const int j = gl<int32_t>(bjob)[tb];
+    const int j = gl<int32_t>(bjob)[2 * blockIdx.x];
+    if (j < 0) return;
+    const int tb = gl<int32_t>(bjob)[2 * blockIdx.x + 1];
     const GemmJob& J = jobs[j];
     if (J.st->done) return;
     const DesignDev& D = dd[J.design];
@@ -363,11 +366,13 @@ __device__ inline void frag_direct(int kind, const void* p, const void* plo, int
 
 __global__ __launch_bounds__(256) void k_gemm_mfma_sk(const GemmJob* jobs, const int32_t* bjob, const DesignDev* dd) {
     __shared__ float red[4][64][65];
-    const int j = gl<int32_t>(bjob)[blockIdx.x];
+    const int j = gl<int32_t>(bjob)[2 * blockIdx.x];
+    if (j < 0) return;
+    const int tb = gl<int32_t>(bjob)[2 * blockIdx.x + 1];
     const GemmJob& J = jobs[j];
     if (J.st->done) return;
     const DesignDev& D = dd[J.design];
-    const int t = blockIdx.x - J.tile_start;
+    const int t = tb - J.tile_start;
     const int m0 = (t / J.tiles_n) * 64, n0 = (t % J.tiles_n) * 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const void* pa = J.a_kind ? J.A_hi : J.A;

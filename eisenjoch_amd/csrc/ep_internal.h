@@ -23,6 +23,7 @@ constexpr double kFixScale = 4294967296.0;  // 2^32 fixed point for deterministi
 constexpr int kScatCells = 1024;     // cells per scatter block
 constexpr int kScatBins = 6144;      // LDS accumulation window of a scatter block (bins)
 constexpr int kHfStat = 16;      // per high-fanout net: 4 extremes, 8 sums, weight, pad
+constexpr int kXcds = 8;          // MI355X: blocks are dealt to the 8 XCDs in turn
 constexpr int kSplitMinK = 512;   // K from which transform products are split
 constexpr int kSplitSlice = 128;  // K per slice of a split product
 
@@ -208,7 +209,7 @@ void launch_wa_reduce(const BatchDev& b, int prec, int total_cell_blocks, double
 // Runs one pass (0 F1, 1 F2, 2 I1, 3 I2; all jobs) of the spectral solve
 // on one engine (0 MFMA, 1 FMA, 2 MFMA split-K, 3 MFMA K stages of 64, 4 K slices).
 void launch_plane_sum(const PlaneSum* sums, int n_sums, int max_blocks, hipStream_t s);
-// bjob: the job of each block (total_tiles entries).
+// bjob: (job, tile) of each block of the grid (total_tiles pairs; -1: none).
 void launch_gemm_pass(const GemmJob* jobs_dev, const int32_t* bjob, int total_tiles,
                       const DesignDev* dd, int prec, int engine, int pass, hipStream_t s);
 

@@ -169,20 +169,60 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     b.tile_blocks = pt[nd];
     // device image: descriptors, prefixes, jobs
     size_t bytes = align_up(sizeof(DesignDev) * nd) + 6 * align_up(sizeof(int32_t) * (nd + 1)) + align_up(4);
+    // Block map of each pass: the dispatcher deals consecutive blocks to the
+    // kXcds XCDs in turn, so block b runs on XCD b % kXcds.  The tiles that
+    // share a panel of a product's data operand (a row of output tiles when
+    // the data is A, in F1 and I1; a column when it is B, in F2 and I2; per K
+    // slice) go whole to one XCD, dealt largest first to the least loaded
+    // XCD (work = tiles x K stages): the panel is then fetched into one L2
+    // and the XCDs stay balanced.  Entries are (job, tile), or (-1, -1) for
+    // the blocks that pad the XCDs to equal length.
     std::vector<int32_t> bjob[4][kEngines];
     for (int p = 0; p < 4; p++)
         for (int e = 0; e < kEngines; e++) {
+            struct Row { int32_t job, first, n, step; int64_t w; };  // tiles first + step * t, t < n
+            std::vector<Row> rows;
+            const bool by_col = (p & 1) != 0;
             int t0 = 0;
             for (size_t q = 0; q < jobs[p][e].size(); q++) {
                 GemmJob& J = jobs[p][e][q];
                 J.tile_start = t0;
-                t0 += ((J.M + 63) / 64) * J.tiles_n * J.ksplit;
-                bjob[p][e].resize(t0, (int32_t)q);
+                const int tm = (J.M + 63) / 64, tiles = tm * J.tiles_n;
+                const int64_t stages = std::max(1, J.K / J.ksplit / 32);
+                for (int ks = 0; ks < J.ksplit; ks++) {
+                    if (by_col)
+                        for (int n = 0; n < J.tiles_n; n++)
+                            rows.push_back({(int32_t)q, t0 + ks * tiles + n, tm, J.tiles_n, (int64_t)tm * stages});
+                    else
+                        for (int m = 0; m < tm; m++)
+                            rows.push_back({(int32_t)q, t0 + ks * tiles + m * J.tiles_n, J.tiles_n, 1,
+                                            (int64_t)J.tiles_n * stages});
+                }
+                t0 += tiles * J.ksplit;
             }
-            b.tiles2[p][e] = t0;
+            std::stable_sort(rows.begin(), rows.end(), [](const Row& a, const Row& r) { return a.w > r.w; });
+            std::vector<std::vector<int32_t>> lists(kXcds);
+            std::vector<int64_t> load(kXcds, 0);
+            for (const Row& r : rows) {
+                const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                load[x] += r.w;
+                for (int t = 0; t < r.n; t++) {
+                    lists[x].push_back(r.job);
+                    lists[x].push_back(r.first + t * r.step);
+                }
+            }
+            size_t len = 0;
+            for (auto& l : lists) len = std::max(len, l.size() / 2);
+            bjob[p][e].assign(2 * len * kXcds, -1);
+            for (int x = 0; x < kXcds; x++)
+                for (size_t s = 0; s < lists[x].size() / 2; s++) {
+                    bjob[p][e][2 * (s * kXcds + x)] = lists[x][2 * s];
+                    bjob[p][e][2 * (s * kXcds + x) + 1] = lists[x][2 * s + 1];
+                }
+            b.tiles2[p][e] = (int)(len * kXcds);
             b.njobs2[p][e] = (int)jobs[p][e].size();
             bytes += align_up(sizeof(GemmJob) * std::max<size_t>(1, jobs[p][e].size()));
-            bytes += align_up(sizeof(int32_t) * std::max<size_t>(1, bjob[p][e].size()));
+            bytes += align_up(sizeof(int32_t) * std::max<size_t>(2, bjob[p][e].size()));
         }
     bytes += align_up(sizeof(PlaneSum) * std::max<size_t>(1, sums.size()));
     // block -> design tables of the six grids
@@ -224,7 +264,8 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             memset(&dummy, 0, sizeof(dummy));
             if (jobs[p][e].empty()) b.jobs2[p][e] = (GemmJob*)put(&dummy, sizeof(GemmJob));
             else b.jobs2[p][e] = (GemmJob*)put(jobs[p][e].data(), sizeof(GemmJob) * jobs[p][e].size());
-            if (bjob[p][e].empty()) b.bjob2[p][e] = (int32_t*)put(&zero, sizeof(int32_t));
+            const int32_t none[2] = {-1, -1};
+            if (bjob[p][e].empty()) b.bjob2[p][e] = (int32_t*)put(none, sizeof(none));
             else b.bjob2[p][e] = (int32_t*)put(bjob[p][e].data(), sizeof(int32_t) * bjob[p][e].size());
         }
     {
