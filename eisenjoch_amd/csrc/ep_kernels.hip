@@ -159,8 +159,11 @@ __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
     const int w = (blockIdx.x - b.pfx_wa[d]) * 4 + (threadIdx.x >> 6);
     if (w >= D.n_wa_waves) return;  // uniform per wave; no block barrier below
     const int lane = threadIdx.x & 63;
+    // the wave's degree bucket: the boundaries at or below w, counted (the
+    // starts are non-decreasing), so the boundary loads do not wait on each other
     int bk = 0;
-    while (bk + 1 < D.n_buckets && w >= D.bucket_wave_start[bk + 1]) bk++;
+#pragma unroll
+    for (int q = 1; q < kMaxBuckets; q++) bk += (q < D.n_buckets && w >= D.bucket_wave_start[q]) ? 1 : 0;
     const int G = D.bucket_group[bk];
     const uint32_t c = D.slots[(size_t)w * kWave + lane];
     const bool valid = c != kPad;
