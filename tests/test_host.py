@@ -222,3 +222,19 @@ def test_bench_roofline_inputs():
     assert bench.group_traffic(f.name, bench.KERNEL_GROUPS["wa"]) == 1024 * (24 + 3)
     assert bench.group_traffic(f.name, bench.KERNEL_GROUPS["gather+update"]) is None
     assert bench.group_traffic("/nonexistent.json", bench.KERNEL_GROUPS["wa"]) is None
+
+
+def test_bench_kernel_groups_match_the_committed_profile():
+    # every kernel pattern of a roofline group names a kernel of the committed
+    # counter summary (a renamed or re-templated kernel would otherwise drop
+    # the group's traffic to null without notice)
+    import json
+    import sys as _sys
+    _sys.path.insert(0, str(ROOT))
+    import bench
+    t = json.loads((ROOT / "profiles" / "pmc_traffic_batched64.json").read_text())
+    names = list(t["kernels"])
+    for g in bench.KERNEL_GROUPS.values():
+        for prefix in g["kernels"]:
+            assert any(prefix in n for n in names), prefix
+        assert bench.group_traffic(str(ROOT / "profiles" / "pmc_traffic_batched64.json"), g) > 0
