@@ -186,8 +186,23 @@ __device__ inline void load_stage(Stage<TK>& s, const float* pd, int rd, const _
     }
 }
 
+// LDS row layout of a stage.  TK = 32: rows of 64 bytes (four 16-byte
+// chunks of 8 BF16) with chunk c of row r stored at chunk c ^ ((r / 4) % 4),
+// so that in units of 16 bytes (sixteen per pass over the 64 banks) the
+// stage writes (four lanes per row, four consecutive rows per 16 lanes) and
+// the MFMA operand reads (16 consecutive rows, one chunk) each touch 16
+// distinct units: no bank conflicts.  Otherwise rows are padded by 8 BF16.
 template <int TK>
-__device__ inline void put_stage(__bf16 (*sd)[64][TK + 8], __bf16 (*sc)[64][TK + 8],
+constexpr int lds_row() { return TK == 32 ? TK : TK + 8; }
+
+template <int TK>
+__device__ inline int lds_at(int row, int k) {  // element offset of (row, k) within its row
+    if constexpr (TK == 32) return (((k >> 3) ^ ((row >> 2) & 3)) << 3) | (k & 7);
+    else return k;
+}
+
+template <int TK>
+__device__ inline void put_stage(__bf16 (*sd)[64][lds_row<TK>()], __bf16 (*sc)[64][lds_row<TK>()],
                                  const Stage<TK>& s) {
     const int row = threadIdx.x >> 2, kc = (threadIdx.x & 3) * 8;
 #pragma unroll
@@ -196,24 +211,26 @@ __device__ inline void put_stage(__bf16 (*sd)[64][TK + 8], __bf16 (*sc)[64][TK +
                             s.d1[c][0], s.d1[c][1], s.d1[c][2], s.d1[c][3]};
         bf16x8 hi, lo;
         split8(v, hi, lo);
-        *(bf16x8*)&sd[0][row][kc + 32 * c] = hi;
-        *(bf16x8*)&sd[1][row][kc + 32 * c] = lo;
-        *(bf16x8*)&sc[0][row][kc + 32 * c] = s.hi[c];
-        *(bf16x8*)&sc[1][row][kc + 32 * c] = s.lo[c];
+        const int at = lds_at<TK>(row, kc + 32 * c);
+        *(bf16x8*)&sd[0][row][at] = hi;
+        *(bf16x8*)&sd[1][row][at] = lo;
+        *(bf16x8*)&sc[0][row][at] = s.hi[c];
+        *(bf16x8*)&sc[1][row][at] = s.lo[c];
     }
 }
 
 // The three split products hi*hi + hi*lo + lo*hi of one LDS stage into the
 // wave's 32 x 32 quadrant.
 template <int TK>
-__device__ inline void mma_stage(f32x16& acc, __bf16 (*sa)[64][TK + 8], __bf16 (*sb)[64][TK + 8],
-                                 int ra, int rb, int kh) {
+__device__ inline void mma_stage(f32x16& acc, __bf16 (*sa)[64][lds_row<TK>()],
+                                 __bf16 (*sb)[64][lds_row<TK>()], int ra, int rb, int kh) {
 #pragma unroll
     for (int kk = 0; kk < TK; kk += 16) {
-        const bf16x8 ah = *(const bf16x8*)&sa[0][ra][kk + kh];
-        const bf16x8 al = *(const bf16x8*)&sa[1][ra][kk + kh];
-        const bf16x8 bh = *(const bf16x8*)&sb[0][rb][kk + kh];
-        const bf16x8 bl = *(const bf16x8*)&sb[1][rb][kk + kh];
+        const int aa = lds_at<TK>(ra, kk + kh), ab = lds_at<TK>(rb, kk + kh);
+        const bf16x8 ah = *(const bf16x8*)&sa[0][ra][aa];
+        const bf16x8 al = *(const bf16x8*)&sa[1][ra][aa];
+        const bf16x8 bh = *(const bf16x8*)&sb[0][rb][ab];
+        const bf16x8 bl = *(const bf16x8*)&sb[1][rb][ab];
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
@@ -221,18 +238,18 @@ __device__ inline void mma_stage(f32x16& acc, __bf16 (*sa)[64][TK + 8], __bf16 (
 }
 
 // 64 x 64 output tile per block, four waves each owning a 32 x 32 quadrant,
-// K in stages of TK through NBUF LDS buffers (rows padded by 8 BF16).  Two
+// K in stages of TK through NBUF LDS buffers (rows laid out by lds_at).  Two
 // register sets alternate (the loop is unrolled by two, so no register copy
 // waits on a load in flight): while a stage is multiplied, the loads of the
 // next two stages are outstanding.  Past the last stage the loads repeat the
 // last stage (in bounds, unused).  TK = 64 with one buffer halves the chain
-// of stages in the LDS of TK = 32 with two (40 KB: four resident blocks per
-// CU).  ADATA: the data operand is A (F1, I1), else B (F2, I2).
+// of stages in about the LDS of TK = 32 with two (40 and 32 KB: four
+// resident blocks per CU).  ADATA: the data operand is A (F1, I1), else B (F2, I2).
 // Requires M, N multiples of 64 and K a multiple of TK (checked on the host).
 template <int TK, int NBUF, bool ADATA, bool SUM>
 __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, const int32_t* bjob, const DesignDev* dd) {
-    __shared__ __bf16 sA[NBUF][2][64][TK + 8];  // [buffer][hi/lo][row][k]
-    __shared__ __bf16 sB[NBUF][2][64][TK + 8];
+    __shared__ __bf16 sA[NBUF][2][64][lds_row<TK>()];  // [buffer][hi/lo][row][k (laid out by lds_at)]
+    __shared__ __bf16 sB[NBUF][2][64][lds_row<TK>()];
     constexpr int b1 = NBUF - 1;  // the second buffer (the first again when single)
     const int j = gl<int32_t>(bjob)[2 * blockIdx.x];
     if (j < 0) return;
@@ -255,8 +272,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, const in
     const int rd = ADATA ? m0 : n0, rc = ADATA ? n0 : m0;
     const int parts = ADATA ? J.a_parts : J.b_parts;
     const int64_t pstride = ADATA ? J.a_pstride : J.b_pstride;
-    __bf16(*sD)[2][64][TK + 8] = ADATA ? sA : sB;
-    __bf16(*sC)[2][64][TK + 8] = ADATA ? sB : sA;
+    __bf16(*sD)[2][64][lds_row<TK>()] = ADATA ? sA : sB;
+    __bf16(*sC)[2][64][lds_row<TK>()] = ADATA ? sB : sA;
     f32x16 acc;
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = 0.0f;
