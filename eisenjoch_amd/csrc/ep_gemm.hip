@@ -335,6 +335,27 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmJob* jobs, const in
         }
         return;
     }
+    if (J.epi == kEpiStoreT) {
+        // transposed output (C[col][row]): the tile goes through LDS (free
+        // after the last stage; at [col][row ^ col], so that both the
+        // quadrant writes and the column reads below hit 64 distinct banks)
+        // and each wave then stores whole 256-byte columns, one per
+        // instruction, instead of 64 scattered words
+        float* T = (float*)&sA[0][0][0][0];
+        __syncthreads();  // an odd last stage reads buffer 0 after the loop
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int r = wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3), c = col - n0;
+            T[c * 64 + (r ^ c)] = acc[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int c = wv * 16 + q;
+            gw<float>(J.C)[po + (size_t)(n0 + c) * J.M + m0 + lane] = T[c * 64 + (lane ^ c)];
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int row = m0 + wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
