@@ -180,36 +180,60 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     std::vector<int32_t> bjob[4][kEngines];
     for (int p = 0; p < 4; p++)
         for (int e = 0; e < kEngines; e++) {
-            struct Row { int32_t job, first, n, step; int64_t w; };  // tiles first + step * t, t < n
-            std::vector<Row> rows;
+            // a unit: the (job, tile) pairs that share a data panel, and its work
+            struct Unit { std::vector<int32_t> jt; int64_t w; };
+            std::vector<Unit> units;
             const bool by_col = (p & 1) != 0;
             int t0 = 0;
+            std::vector<int> start(jobs[p][e].size());
             for (size_t q = 0; q < jobs[p][e].size(); q++) {
                 GemmJob& J = jobs[p][e][q];
                 J.tile_start = t0;
+                start[q] = t0;
+                t0 += ((J.M + 63) / 64) * J.tiles_n * J.ksplit;
+            }
+            for (size_t q = 0; q < jobs[p][e].size(); q++) {
+                const GemmJob& J = jobs[p][e][q];
                 const int tm = (J.M + 63) / 64, tiles = tm * J.tiles_n;
                 const int64_t stages = std::max(1, J.K / J.ksplit / 32);
+                // I2 writes Ex and Ey interleaved from two jobs of one design
+                // (pushed one after the other): their tiles of a column form one
+                // unit, so the two halves of each output line are written by
+                // neighbouring blocks of one XCD and merge in its L2
+                const bool pair = p == 3 && J.epi == kEpiStore2 && q + 1 < jobs[p][e].size() &&
+                                  jobs[p][e][q + 1].design == J.design && jobs[p][e][q + 1].epi == kEpiStore2;
                 for (int ks = 0; ks < J.ksplit; ks++) {
                     if (by_col)
-                        for (int n = 0; n < J.tiles_n; n++)
-                            rows.push_back({(int32_t)q, t0 + ks * tiles + n, tm, J.tiles_n, (int64_t)tm * stages});
+                        for (int n = 0; n < J.tiles_n; n++) {
+                            Unit u{{}, 0};
+                            for (int m = 0; m < tm; m++)
+                                for (size_t h = 0; h < (pair ? 2u : 1u); h++) {
+                                    u.jt.push_back((int32_t)(q + h));
+                                    u.jt.push_back(start[q + h] + ks * tiles + m * J.tiles_n + n);
+                                    u.w += stages;
+                                }
+                            units.push_back(std::move(u));
+                        }
                     else
-                        for (int m = 0; m < tm; m++)
-                            rows.push_back({(int32_t)q, t0 + ks * tiles + m * J.tiles_n, J.tiles_n, 1,
-                                            (int64_t)J.tiles_n * stages});
+                        for (int m = 0; m < tm; m++) {
+                            Unit u{{}, 0};
+                            for (int n = 0; n < J.tiles_n; n++) {
+                                u.jt.push_back((int32_t)q);
+                                u.jt.push_back(start[q] + ks * tiles + m * J.tiles_n + n);
+                                u.w += stages;
+                            }
+                            units.push_back(std::move(u));
+                        }
                 }
-                t0 += tiles * J.ksplit;
+                if (pair) q++;
             }
-            std::stable_sort(rows.begin(), rows.end(), [](const Row& a, const Row& r) { return a.w > r.w; });
+            std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& r) { return a.w > r.w; });
             std::vector<std::vector<int32_t>> lists(kXcds);
             std::vector<int64_t> load(kXcds, 0);
-            for (const Row& r : rows) {
+            for (const Unit& u : units) {
                 const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-                load[x] += r.w;
-                for (int t = 0; t < r.n; t++) {
-                    lists[x].push_back(r.job);
-                    lists[x].push_back(r.first + t * r.step);
-                }
+                load[x] += u.w;
+                lists[x].insert(lists[x].end(), u.jt.begin(), u.jt.end());
             }
             size_t len = 0;
             for (auto& l : lists) len = std::max(len, l.size() / 2);
