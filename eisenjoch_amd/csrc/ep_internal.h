@@ -24,6 +24,9 @@ constexpr int kScatCells = 1024;     // cells per scatter block
 constexpr int kScatBins = 6144;      // LDS accumulation window of a scatter block (bins)
 constexpr int kHfStat = 16;      // per high-fanout net: 4 extremes, 8 sums, weight, pad
 constexpr int kXcds = 8;          // MI355X: blocks are dealt to the 8 XCDs in turn
+constexpr int kWaUnits = 2;       // 64-slot net units per wave of the WA kernel
+// blocks (four waves) of the WA kernel for a design of n_wa_waves units
+inline int wa_blocks_of(int n_wa_waves) { return (n_wa_waves + 4 * kWaUnits - 1) / (4 * kWaUnits); }
 constexpr int kSplitMinK = 512;   // K from which transform products are split
 constexpr int kSplitSlice = 128;  // K per slice of a split product
 
@@ -80,9 +83,11 @@ struct DesignDev {
     ep_iter_rec* log;          // [max_iters]
     // cells
     void* x0[2];               // initial positions (reset source)
-    void* pos[2];              // x, y (solver state)
+    void* pos[2];              // x, y: the solver state [n_mov], then the fixed cells' locations
+                               // [n_fixed], so a pin of any cell reads pos[ax][cell]
     void* dpos[2];             // x - x_prev
-    void* look;                // T [n_mov][2]: clamped look-ahead, x and y interleaved
+    void* look;                // T [n_mov + n_fixed][2]: clamped look-ahead, x and y interleaved,
+                               // then the fixed cells' locations
     void* g[2][2];             // [parity][axis] combined gradient (or density grad)
     void* pg;                  // per-slot WA gradient (gx, gy) in T [n_slots][2]
     uint32_t* cs_off;          // cell -> slot CSR [n_mov + 1]
@@ -104,7 +109,6 @@ struct DesignDev {
     uint32_t* tofs;            // [n_mov] offset of perm[k&1][j] within its tile, by j
     void* pinw;                // T[n_mov]
     float* hs;                 // half side of the footprint [n_mov]
-    void* fixed[2];            // T[n_fixed]
     // nets
     uint32_t* slots;           // padded pin slots, n_wa_waves * 64
     uint32_t* hf_off;          // [n_hf + 1]  (nets with > 64 pins)

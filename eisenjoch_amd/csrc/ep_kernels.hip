@@ -109,66 +109,52 @@ template <> struct Vec2<float> { typedef float2 type; };
 template <> struct Vec2<double> { typedef double2 type; };
 
 // Position of pin cell c: the interleaved look-ahead (WA) or the solver
-// positions (HPWL) of a movable cell, else the fixed cell's location.
+// positions (HPWL); the fixed cells' locations follow the movable cells'
+// entries in both, so any cell is one load without a branch.
 template <typename T, bool HPWL>
 __device__ inline void pin_pos(const DesignDev& D, uint32_t c, T& px, T& py) {
-    if (c < (uint32_t)D.n_mov) {
-        if (HPWL) {
-            px = ((const T*)D.pos[0])[c];
-            py = ((const T*)D.pos[1])[c];
-        } else {
-            const typename Vec2<T>::type v = ((const typename Vec2<T>::type*)D.look)[c];
-            px = v.x;
-            py = v.y;
-        }
-    } else {
-        px = ((const T*)D.fixed[0])[c - D.n_mov];
-        py = ((const T*)D.fixed[1])[c - D.n_mov];
-    }
-}
-
-// Per-pin gradients go to their own slot; each cell later sums its slots in
-// a fixed order (cell -> slot CSR), so there are no atomics and the sum is
-// deterministic.
-template <typename T>
-__device__ inline void store_grad(const DesignDev& D, uint32_t slot, T gx, T gy) {
-    typedef typename Vec2<T>::type V;
-    if (D.slot_w) {  // net weight, common.rs:246-261 (absent: 1)
-        const T w = ((const T*)D.slot_w)[slot];
-        gx = w * gx;
-        gy = w * gy;
-    }
-    V v;
-    v.x = gx;
-    v.y = gy;
-    ((V*)D.pg)[slot] = v;
-}
-
-// HPWL mode: per-wave partial of sum over nets of (max - min) x + y of the
-// solver positions; else the WA gradient of the look-ahead positions.
-template <typename T, bool FIX, bool HPWL>
-__global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
-    const int d = design_of(b.blk_wa, b.pfx_wa, b.nd, blockIdx.x);
-    const DesignDev& D = b.d[d];
-    const DevState* st = D.st;
     if (HPWL) {
-        if (!final_pass && !(st->run_update && st->check && D.cfg.log_hpwl)) return;
-    } else if (st->done) {
-        return;
+        px = ((const T*)D.pos[0])[c];
+        py = ((const T*)D.pos[1])[c];
+    } else {
+        const typename Vec2<T>::type v = ((const typename Vec2<T>::type*)D.look)[c];
+        px = v.x;
+        py = v.y;
     }
-    const int w = (blockIdx.x - b.pfx_wa[d]) * 4 + (threadIdx.x >> 6);
-    if (w >= D.n_wa_waves) return;  // uniform per wave; no block barrier below
-    const int lane = threadIdx.x & 63;
-    // the wave's degree bucket: the boundaries at or below w, counted (the
-    // starts are non-decreasing), so the boundary loads do not wait on each other
+}
+
+// What a WA unit reads of its design's descriptor, loaded once at kernel
+// start: loads after a store of the same kernel cannot be scalar loads.
+template <typename T>
+struct WaArgs {
+    T a;                  // wl_coeff
+    uint32_t n_mov;
+    void* pg;
+    const void* slot_w;
+    double* part_net;
+};
+
+// Group size (pins per net) of unit w: the bucket boundaries at or below w,
+// counted (the starts are non-decreasing), so the boundary loads do not wait
+// on each other.
+__device__ inline int wa_group(const DesignDev& D, int w) {
     int bk = 0;
 #pragma unroll
     for (int q = 1; q < kMaxBuckets; q++) bk += (q < D.n_buckets && w >= D.bucket_wave_start[q]) ? 1 : 0;
-    const int G = D.bucket_group[bk];
-    const uint32_t c = D.slots[(size_t)w * kWave + lane];
+    return D.bucket_group[bk];
+}
+
+// One 64-slot unit w of the net layout (nets of G pins): the group
+// reductions of its nets.  HPWL mode: h (lane 0) is the unit's partial of sum
+// over nets of (max - min) x + y of the solver positions.  Else (gx, gy) is
+// the WA gradient of the lane's pin, to be stored to the pin's own slot when
+// the lane holds a movable pin; each cell later sums its slots in a fixed
+// order (cell -> slot CSR), so there are no atomics and the sum is
+// deterministic.
+template <typename T, bool HPWL>
+__device__ inline void wa_unit(const WaArgs<T>& A, int G, int lane, uint32_t c, T px, T py, double& h,
+                               T& gx, T& gy) {
     const bool valid = c != kPad;
-    T px = 0, py = 0;
-    if (valid) pin_pos<T, HPWL>(D, c, px, py);
     const T big = Lim<T>::big();
     T xmx = valid ? px : -big, xmn = valid ? px : big;
     T ymx = valid ? py : -big, ymn = valid ? py : big;
@@ -181,13 +167,11 @@ __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
     if (HPWL) {
         // wave padding forms whole groups without a valid pin (max < min): skip them
         const bool has = xmx >= xmn;
-        double h = ((lane & (G - 1)) == 0 && has)
-                       ? ((double)xmx - (double)xmn) + ((double)ymx - (double)ymn) : 0.0;
+        h = ((lane & (G - 1)) == 0 && has) ? ((double)xmx - (double)xmn) + ((double)ymx - (double)ymn) : 0.0;
         for (int o = 32; o >= 1; o >>= 1) h += __shfl_xor(h, o, kWave);
-        if (lane == 0) D.part_net[w] = h;
         return;
     }
-    const T a = (T)D.cfg.wl_coeff;
+    const T a = A.a;
     const T ux = px - xmx, vx = px - xmn, uy = py - ymx, vy = py - ymn;
     const T ex = valid ? fexp(a * ux) : T(0), fx = valid ? fexp(-a * vx) : T(0);
     const T ey = valid ? fexp(a * uy) : T(0), fy = valid ? fexp(-a * vy) : T(0);
@@ -195,10 +179,63 @@ __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
     for (int o = 1; o < G; o <<= 1)
 #pragma unroll
         for (int q = 0; q < 8; q++) s[q] += shfl_xor(s[q], o);
-    if (valid && c < (uint32_t)D.n_mov) {
-        T gx = wa_pin<T>(a, ux, ex, s[0], s[1], vx, fx, s[2], s[3]);
-        T gy = wa_pin<T>(a, uy, ey, s[4], s[5], vy, fy, s[6], s[7]);
-        store_grad<T>(D, (uint32_t)w * kWave + lane, gx, gy);
+    gx = wa_pin<T>(a, ux, ex, s[0], s[1], vx, fx, s[2], s[3]);
+    gy = wa_pin<T>(a, uy, ey, s[4], s[5], vy, fy, s[6], s[7]);
+}
+
+// Each wave runs kWaUnits consecutive units: the slot loads of all of them,
+// then their position loads, are issued before any unit's reductions, and
+// every store follows all of the reductions, so a wave keeps kWaUnits
+// dependent load chains in flight and never waits on its own stores.
+template <typename T, bool FIX, bool HPWL>
+__global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
+    const int d = design_of(b.blk_wa, b.pfx_wa, b.nd, blockIdx.x);
+    const DesignDev& D = b.d[d];
+    const DevState* st = D.st;
+    if (HPWL) {
+        if (!final_pass && !(st->run_update && st->check && D.cfg.log_hpwl)) return;
+    } else if (st->done) {
+        return;
+    }
+    // the wave index through readfirstlane: the unit, its bucket and group
+    // size are then scalars and the descriptor fields they select scalar loads
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int w0 = ((blockIdx.x - b.pfx_wa[d]) * 4 + wv) * kWaUnits;
+    if (w0 >= D.n_wa_waves) return;  // uniform per wave; no block barrier below
+    const int lane = threadIdx.x & 63;
+    const int nu = min(kWaUnits, D.n_wa_waves - w0);
+    const WaArgs<T> A{(T)D.cfg.wl_coeff, (uint32_t)D.n_mov, D.pg, D.slot_w, D.part_net};
+    uint32_t c[kWaUnits];
+    int G[kWaUnits];
+    T px[kWaUnits], py[kWaUnits];
+#pragma unroll
+    for (int u = 0; u < kWaUnits; u++) {
+        c[u] = u < nu ? D.slots[(size_t)(w0 + u) * kWave + lane] : kPad;
+        G[u] = wa_group(D, w0 + u);
+    }
+    // a padding slot loads cell 0's entry (value unused)
+#pragma unroll
+    for (int u = 0; u < kWaUnits; u++) pin_pos<T, HPWL>(D, c[u] != kPad ? c[u] : 0u, px[u], py[u]);
+    double h[kWaUnits];
+    T gx[kWaUnits], gy[kWaUnits];
+#pragma unroll
+    for (int u = 0; u < kWaUnits; u++)
+        if (u < nu) wa_unit<T, HPWL>(A, G[u], lane, c[u], px[u], py[u], h[u], gx[u], gy[u]);
+#pragma unroll
+    for (int u = 0; u < kWaUnits; u++) {
+        if (u >= nu) break;
+        const int w = w0 + u;
+        if (HPWL) {
+            if (lane == 0) A.part_net[w] = h[u];
+        } else if (c[u] != kPad && c[u] < A.n_mov) {
+            const uint32_t slot = (uint32_t)w * kWave + lane;
+            T wt = T(1);
+            if (A.slot_w) wt = ((const T*)A.slot_w)[slot];  // net weight, common.rs:246-261 (absent: 1)
+            typename Vec2<T>::type v;
+            v.x = A.slot_w ? wt * gx[u] : gx[u];
+            v.y = A.slot_w ? wt * gy[u] : gy[u];
+            ((typename Vec2<T>::type*)A.pg)[slot] = v;
+        }
     }
 }
 

@@ -89,7 +89,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         dv[i].cfg = make_cfg(cfgs[i]);
         pc[i + 1] = pc[i] + D.dev.n_cell_blocks;
         pb[i + 1] = pb[i] + D.dev.n_bin_blocks;
-        pw[i + 1] = pw[i] + (D.dev.n_wa_waves + 3) / 4;
+        pw[i + 1] = pw[i] + wa_blocks_of(D.dev.n_wa_waves);
         ph[i + 1] = ph[i] + D.dev.n_hf;
         ps[i + 1] = ps[i] + D.dev.n_scat_blocks;
         pt[i + 1] = pt[i] + D.dev.n_tile_blocks;

@@ -276,9 +276,9 @@ static size_t plan_arena(Design& D, char* base) {
     ArenaPlan a;
     auto P = [&](size_t bytes) -> void* { size_t o = a.take(bytes); return base ? base + o : nullptr; };
     for (int ax = 0; ax < 2; ax++) G.x0[ax] = P(N * T);
-    for (int ax = 0; ax < 2; ax++) G.pos[ax] = P(N * T);
+    for (int ax = 0; ax < 2; ax++) G.pos[ax] = P((N + F) * T);
     for (int ax = 0; ax < 2; ax++) G.dpos[ax] = P(N * T);
-    G.look = P(N * 2 * T);
+    G.look = P((N + F) * 2 * T);
     for (int p = 0; p < 2; p++)
         for (int ax = 0; ax < 2; ax++) G.g[p][ax] = P(N * T);
     G.pg = P((size_t)G.n_slots * 2 * T);
@@ -302,7 +302,6 @@ static size_t plan_arena(Design& D, char* base) {
     G.pinw = P(N * T);
     G.hs = (float*)P(N * 4);
     G.slot_w = D.weighted ? P((size_t)G.n_slots * T) : nullptr;
-    for (int ax = 0; ax < 2; ax++) G.fixed[ax] = P(F * T);
     G.slots = (uint32_t*)P((size_t)G.n_wa_waves * kWave * 4);
     G.hf_off = (uint32_t*)P((size_t)(G.n_hf + 1) * 4);
     G.hf_pins = (uint32_t*)P(D.hf_pin_count * 4);
@@ -528,6 +527,12 @@ int prepare_design(Ctx* c, const ep_design* d, int prec, int log_cap, Prepared& 
     to_prec(P.x0[1], d->y0, N, D.prec);
     to_prec(P.fixed[0], d->fixed_x, F, D.prec);
     to_prec(P.fixed[1], d->fixed_y, F, D.prec);
+    {
+        const size_t Tb = D.prec == EP_PREC_FP64 ? 8 : 4;
+        P.fixed_xy.resize(2 * F * Tb);
+        for (size_t k = 0; k < F; k++)
+            for (int ax = 0; ax < 2; ax++) memcpy(&P.fixed_xy[(2 * k + ax) * Tb], &P.fixed[ax][k * Tb], Tb);
+    }
     // pin weights, common.rs:381-407 (or the caller's)
     std::vector<double> w(N, 0.0);
     if (d->pinw) {
@@ -633,9 +638,13 @@ int commit_design(Ctx* c, Prepared& P, uint32_t* id_out) {
     const NetLayout& L = P.L;
     hipError_t e = hipMemsetAsync(D.arena, 0, P.bytes, c->stream);
     if (e != hipSuccess) rc = fail(c, EP_ERR_DEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    // the fixed cells' locations follow the movable cells' entries of pos
+    // and look (no kernel writes them), so a pin of any cell is one load
+    const size_t Tb = D.prec == EP_PREC_FP64 ? 8 : 4;
+    if (!rc) rc = copy_in(c, (char*)D.dev.look + 2 * N * Tb, P.fixed_xy.data(), P.fixed_xy.size());
     for (int ax = 0; ax < 2 && !rc; ax++) {
         rc = copy_in(c, D.dev.x0[ax], P.x0[ax].data(), P.x0[ax].size());
-        if (!rc) rc = copy_in(c, D.dev.fixed[ax], P.fixed[ax].data(), P.fixed[ax].size());
+        if (!rc) rc = copy_in(c, (char*)D.dev.pos[ax] + N * Tb, P.fixed[ax].data(), P.fixed[ax].size());
         if (!rc) rc = copy_in(c, ax == 0 ? D.dev.spec_x : D.dev.spec_y, P.spec[ax].data(), P.spec[ax].size());
         if (!rc) rc = copy_in(c, D.dev.perm[ax], P.ident.data(), N * 4);
         if (!rc) rc = copy_in(c, D.dev.sorted_hs[ax], P.hs.data(), N * 4);

@@ -136,6 +136,7 @@ struct Prepared {
     size_t bytes = 0;           // arena size
     NetLayout L;
     std::vector<char> x0[2], fixed[2], pinw, spec[2];  // arithmetic type T
+    std::vector<char> fixed_xy;  // fixed locations, x and y interleaved (tail of look)
     std::vector<char> slot_w;   // T per slot: its net's weight (empty: all nets weigh 1)
     std::vector<float> hs;
     std::vector<uint32_t> ident;

@@ -74,7 +74,7 @@ static int finalize_one(Ctx* c, Batch& b, int i) {
     const Design& D = *b.ds[i];
     BatchDev one = single_view(b, i);
     launch_ctrl(one, 2, c->stream);
-    launch_hpwl(one, b.prec, (D.dev.n_wa_waves + 3) / 4, D.dev.n_hf, 1, c->stream);
+    launch_hpwl(one, b.prec, wa_blocks_of(D.dev.n_wa_waves), D.dev.n_hf, 1, c->stream);
     launch_ctrl(one, 3, c->stream);
     hipError_t e = hipMemcpyAsync(&c->state_host[D.state_idx], D.dev.st, sizeof(DevState),
                                   hipMemcpyDeviceToHost, c->stream);
