@@ -421,11 +421,14 @@ static void enqueue_phase(Ctx* c, Batch& b, int ph) {
 // unset.
 static void enqueue_iteration(Ctx* c, Batch& b, bool with_fix) {
     hipEventRecord(c->ev_fork, c->stream);
+    // the density chain's first kernels are enqueued (captured) before the
+    // side stream's, so the chain does not wait behind the fork
+    enqueue_phase(c, b, 0);
     hipStreamWaitEvent(c->side, c->ev_fork, 0);
     launch_wa(b.bd, b.prec, b.wa_blocks, b.hf_blocks, c->side);
     launch_wl_cell(b.bd, b.prec, b.cell_blocks, c->side);
     hipEventRecord(c->ev_join, c->side);
-    for (int ph = 0; ph < kNumPhases; ph++) {
+    for (int ph = 1; ph < kNumPhases; ph++) {
         if (ph == kWaPhase) continue;
         if (!with_fix && (ph == kFixPhase || ph == kCtrlEndPhase)) continue;
         if (ph == kGatherPhase) hipStreamWaitEvent(c->stream, c->ev_join, 0);
