@@ -771,8 +771,11 @@ __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy)
     typedef typename Vec2<T>::type V;
     const V* pg = (const V*)D.pg;
     double sx = 0.0, sy = 0.0;
+    // both CSR bounds in one round of loads (the high-fanout bounds are not
+    // left to wait behind the slot sums)
     uint32_t k = D.cs_off[i];
     const uint32_t e = D.cs_off[i + 1];
+    const uint32_t h0 = D.chf_off[i], h1 = D.chf_off[i + 1];
     if (e > k) {
         // the first eight pins (most cells have no more) in one round: all
         // their slot indices, then all their slot values, are loaded at once
@@ -811,7 +814,6 @@ __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy)
         if (rest > 1) { sx += (double)v1.x; sy += (double)v1.y; }
         if (rest > 2) { sx += (double)v2.x; sy += (double)v2.y; }
     }
-    const uint32_t h0 = D.chf_off[i], h1 = D.chf_off[i + 1];
     if (h1 > h0) {
         const V l = ((const V*)D.look)[i];
         const T a = (T)D.cfg.wl_coeff;
