@@ -56,6 +56,8 @@ struct DevState {
     int32_t prev_check;
     int32_t n_log;
     int32_t pad0;
+    uint64_t t_start;     // device wall clock at the placement's reset
+    uint64_t t_done;      // device wall clock at the control step that stopped the design
 };
 
 // Constant per-design configuration (copied into the descriptor).

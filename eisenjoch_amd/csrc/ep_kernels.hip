@@ -1108,7 +1108,10 @@ __device__ inline void ctrl_end(const DesignDev& D, DevState* st, double* sh) {
     st->prev_check = st->check;
     st->it = k + 1;
     st->n_log = k + 1;
-    if (stop) st->done = 1;
+    if (stop) {
+        st->done = 1;
+        st->t_done = wall_clock64();
+    }
 }
 
 // which = 0: after the gather; 1: after the fix (end of control); 4: both
@@ -1204,6 +1207,7 @@ __global__ __launch_bounds__(256) void k_reset(BatchDev b) {
     const DesignDev& D = b.d[d];
     const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
+    if (i == 0) D.st->t_start = wall_clock64();  // the placement's start, for its latency
     D.perm[0][i] = (uint32_t)i;
     D.sorted_hs[0][i] = D.hs[i];
     T lk[2];

@@ -451,9 +451,9 @@ static bool batch_needs_fix(const Ctx* c, const Batch& b) {
     return false;
 }
 
-int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph) {
+int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph, int fix) {
     if (b.ds.empty() || iters <= 0) return EP_OK;
-    const bool with_fix = batch_needs_fix(c, b);
+    const bool with_fix = fix < 0 ? batch_needs_fix(c, b) : fix != 0;
     if (!use_graph) {
         for (int i = 0; i < iters; i++) enqueue_iteration(c, b, with_fix);
         HIPCHK(c, hipGetLastError());
@@ -510,29 +510,60 @@ void fill_result(const Design& D, const DevState& s, ep_result* r) {
     r->lambda = s.lam;
 }
 
-// Runs a built batch until every design stops.
+// Runs a built batch until every design stops.  The next chunk of
+// iterations is always queued before the host waits for the states copied
+// behind the current one, so the GPU does not idle while the host checks
+// them; a finished batch runs one chunk of iterations that skip every
+// design.  Each design's time is taken on the device, from the reset to the
+// control step that stopped it, so it does not depend on when the host looks.
 int batch_place(Ctx* c, Batch& b, int sync_every, bool use_graph) {
-    const double t0 = now_ms();
     int rc = batch_reset(c, b, nullptr);
     if (rc) return rc;
     int max_it = 0;
     for (auto& cf : b.cfgs) max_it = std::max(max_it, cf.max_iters);
-    int done_iters = 0;
     b.done_ms.assign(b.ds.size(), -1.0);
-    for (;;) {
-        bool all = true;
-        if ((rc = read_states(c, b))) return rc;
-        const double t = now_ms() - t0;
-        for (size_t k = 0; k < b.ds.size(); k++) {
-            const bool dn = c->state_host[b.ds[k]->state_idx].done;
-            if (dn && b.done_ms[k] < 0) b.done_ms[k] = t;
-            all = all && dn;
-        }
-        if (all || done_iters >= max_it) break;
-        if ((rc = batch_run_iters(c, b, sync_every, use_graph))) return rc;
-        done_iters += sync_every;
+    int lo = kStatePool, hi = -1;
+    for (auto* D : b.ds) {
+        lo = std::min(lo, D->state_idx);
+        hi = std::max(hi, D->state_idx);
     }
-    return batch_finalize(c, b);
+    auto all_done = [&]() {
+        for (const Design* D : b.ds)
+            if (!c->state_host[D->state_idx].done) return false;
+        return true;
+    };
+    auto copy_states = [&]() -> int {
+        HIPCHK(c, hipMemcpyAsync(&c->state_host[lo], &c->state_pool[lo], sizeof(DevState) * (hi - lo + 1),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev_states, c->stream));
+        return EP_OK;
+    };
+    int queued = 0;
+    // a penalty once set stays set, so a mirror one chunk old can only ask
+    // for the fix pass where it is no longer needed (it then does nothing)
+    bool fix = batch_needs_fix(c, b);
+    if (!all_done() && max_it > 0) {
+        if ((rc = batch_run_iters(c, b, sync_every, use_graph, fix))) return rc;
+        queued += sync_every;
+        if ((rc = copy_states())) return rc;
+        for (;;) {
+            if (queued < max_it) {
+                if ((rc = batch_run_iters(c, b, sync_every, use_graph, fix))) return rc;
+                queued += sync_every;
+            }
+            HIPCHK(c, hipEventSynchronize(c->ev_states));
+            if (all_done() || queued >= max_it) break;
+            fix = batch_needs_fix(c, b);
+            if ((rc = copy_states())) return rc;
+        }
+    }
+    rc = batch_finalize(c, b);
+    for (size_t k = 0; k < b.ds.size(); k++) {
+        const DevState& s = c->state_host[b.ds[k]->state_idx];
+        b.done_ms[k] = (s.t_done > s.t_start && c->wall_khz > 0.0)
+                           ? (double)(s.t_done - s.t_start) / c->wall_khz : 0.0;
+    }
+    return rc;
 }
 
 Design* find_design(Ctx* c, uint32_t id) {

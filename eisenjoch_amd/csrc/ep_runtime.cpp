@@ -429,6 +429,12 @@ int ep_create(ep_ctx** out, int device, uint64_t budget) {
     HIPCHK(nullptr, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     HIPCHK(nullptr, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIPCHK(nullptr, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    HIPCHK(nullptr, hipEventCreateWithFlags(&c->ev_states, hipEventDisableTiming));
+    {
+        int khz = 0;
+        HIPCHK(nullptr, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+        c->wall_khz = khz > 0 ? (double)khz : 0.0;
+    }
     HIPCHK(nullptr, hipMalloc(&c->state_pool, sizeof(DevState) * kStatePool));
     HIPCHK(nullptr, hipHostMalloc(&c->state_host, sizeof(DevState) * kStatePool, hipHostMallocDefault));
     for (int i = kStatePool - 1; i >= 0; i--) c->free_states.push_back(i);
@@ -457,6 +463,7 @@ void ep_destroy(ep_ctx* h) {
     hipStreamSynchronize(c->side);
     hipEventDestroy(c->ev_fork);
     hipEventDestroy(c->ev_join);
+    hipEventDestroy(c->ev_states);
     hipStreamDestroy(c->side);
     hipStreamDestroy(c->stream);
     delete c;

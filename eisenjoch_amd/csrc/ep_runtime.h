@@ -73,7 +73,7 @@ struct Batch {
     BatchDev bd{};
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // replayed chunk without / with the fix pass
     int graph_iters[2] = {0, 0};
-    std::vector<double> done_ms;     // per design: host time at which it was seen done
+    std::vector<double> done_ms;     // per design: device time from its reset to its stop
 };
 
 struct Server;
@@ -89,6 +89,8 @@ struct Ctx : ep_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;      // WA gradient, concurrent with the density chain
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_states = nullptr;  // the states' copy behind a queued chunk has landed
+    double wall_khz = 0.0;           // device wall-clock rate (wall_clock64 ticks per ms)
     std::recursive_mutex mu;     // serialises device work
     std::mutex err_mu;
     std::string last_error, err_copy;
@@ -162,7 +164,9 @@ void release_batch_cache(Ctx* c);
 int batch_build(Ctx* c, const std::vector<Design*>& ds, const ep_config* cfg, Batch& b);
 void batch_release(Batch& b);
 int batch_reset(Ctx* c, Batch& b, const ep_config* cfg);
-int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph);
+// fix: 1 / 0 launches the iterations with / without the fix pass; -1 decides
+// from the host mirror of the states.
+int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph, int fix = -1);
 int batch_finalize(Ctx* c, Batch& b);
 int read_states(Ctx* c, const Batch& b);
 void fill_result(const Design& D, const DevState& s, ep_result* r);
