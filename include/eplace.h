@@ -126,7 +126,9 @@ typedef struct ep_result {
     double lambda;
     double t_submit_ms;     /* serving: submit, admit and finish times */
     double t_admit_ms;
-    double t_done_ms;
+    double t_done_ms;       /* batch placement: time from the placement's reset
+                               to the iteration that stopped the design, on the
+                               device clock (submit and admit are 0) */
 } ep_result;
 
 /* ---- context ---- */
