@@ -24,6 +24,7 @@ constexpr int kScatCells = 1024;     // cells per scatter block
 constexpr int kScatBins = 6144;      // LDS accumulation window of a scatter block (bins)
 constexpr int kHfStat = 16;      // per high-fanout net: 4 extremes, 8 sums, weight, pad
 constexpr int kXcds = 8;          // MI355X: blocks are dealt to the 8 XCDs in turn
+constexpr int kCellSlots = 8;     // slots of a cell held at a fixed stride (cs8)
 constexpr int kWaUnits = 2;       // 64-slot net units per wave of the WA kernel
 // blocks (four waves) of the WA kernel for a design of n_wa_waves units
 inline int wa_blocks_of(int n_wa_waves) { return (n_wa_waves + 4 * kWaUnits - 1) / (4 * kWaUnits); }
@@ -91,11 +92,13 @@ struct DesignDev {
     void* look;                // T [n_mov + n_fixed][2]: clamped look-ahead, x and y interleaved,
                                // then the fixed cells' locations
     void* g[2][2];             // [parity][axis] combined gradient (or density grad)
-    void* pg;                  // per-slot WA gradient (gx, gy) in T [n_slots][2]
+    void* pg;                  // per-slot WA gradient (gx, gy) in T [n_slots + 1][2]; entry n_slots
+                               // is never written and stays zero
     uint32_t* cs_off;          // cell -> slot CSR [n_mov + 1]
     double* gwl;               // [n_mov][2] WA gradient per cell (its slots summed in FP64)
     const void* slot_w;        // T [n_slots]: each slot's net weight, or null (all nets weigh 1)
     uint32_t* cs_idx;          // slots of each movable cell, ascending (nets of <= 64 pins)
+    uint32_t* cs8;             // [n_mov][kCellSlots]: the first kCellSlots of them, then slot n_slots
     uint32_t* chf_off;         // cell -> high-fanout-net CSR [n_mov + 1]
     uint32_t* chf_net;         // high-fanout nets of each cell, ascending
     void* hf_stat;             // T [n_hf][kHfStat]: extremes, exponential sums, weight

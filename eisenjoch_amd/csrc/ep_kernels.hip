@@ -776,24 +776,25 @@ __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy)
     uint32_t k = D.cs_off[i];
     const uint32_t e = D.cs_off[i + 1];
     const uint32_t h0 = D.chf_off[i], h1 = D.chf_off[i + 1];
-    if (e > k) {
-        // the first eight pins (most cells have no more) in one round: all
-        // their slot indices, then all their slot values, are loaded at once
-        // (indices past the cell's last pin repeat it and are not added)
-        const uint32_t n1 = min(e - k, 8u);
-        uint32_t s8[8];
+    {
+        // the first eight slots (most cells have no more) come from the
+        // fixed-stride copy, loaded with the bounds: their values are the
+        // second round of loads.  Past the cell's last pin the row names the
+        // zero slot, whose +0.0 leaves the sum unchanged bit for bit (it
+        // starts at +0.0 and never becomes -0.0).
+        static_assert(kCellSlots == 8, "two 16-byte loads per cell");
+        const uint4 a = ((const uint4*)D.cs8)[2 * (size_t)i];
+        const uint4 c = ((const uint4*)D.cs8)[2 * (size_t)i + 1];
+        const uint32_t s8[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
         V v8[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) s8[q] = D.cs_idx[k + min((uint32_t)q, n1 - 1)];
 #pragma unroll
         for (int q = 0; q < 8; q++) v8[q] = pg[s8[q]];
 #pragma unroll
-        for (int q = 0; q < 8; q++)
-            if ((uint32_t)q < n1) {
-                sx += (double)v8[q].x;
-                sy += (double)v8[q].y;
-            }
-        k += n1;
+        for (int q = 0; q < 8; q++) {
+            sx += (double)v8[q].x;
+            sy += (double)v8[q].y;
+        }
+        k += min(e - k, 8u);
     }
     for (; k + 4 <= e; k += 4) {
         const uint32_t s0 = D.cs_idx[k], s1 = D.cs_idx[k + 1];

@@ -207,6 +207,13 @@ static void build_layout(const ep_design* d, NetLayout& L) {
         uint32_t c = L.slots[s];
         if (c != kPad && c < nm) L.cs_idx[fillp[c]++] = (uint32_t)s;
     }
+    // each cell's first kCellSlots slots at a fixed stride, the rest of the
+    // row pointing at the zero slot past every real one (n_slots)
+    const uint32_t zero_slot = (uint32_t)(L.slots.size() + L.hf_pins.size());
+    L.cs8.assign((size_t)nm * kCellSlots, zero_slot);
+    for (size_t i = 0; i < nm; i++)
+        for (uint32_t q = 0; q < (uint32_t)kCellSlots && L.cs_off[i] + q < L.cs_off[i + 1]; q++)
+            L.cs8[i * kCellSlots + q] = L.cs_idx[L.cs_off[i] + q];
     // cell -> high-fanout-net CSR (nets ascending, a net once per pin of the cell)
     L.chf_off.assign((size_t)nm + 1, 0);
     const size_t nhf = L.hf_off.size() - 1;
@@ -281,7 +288,8 @@ static size_t plan_arena(Design& D, char* base) {
     G.look = P((N + F) * 2 * T);
     for (int p = 0; p < 2; p++)
         for (int ax = 0; ax < 2; ax++) G.g[p][ax] = P(N * T);
-    G.pg = P((size_t)G.n_slots * 2 * T);
+    G.pg = P(((size_t)G.n_slots + 1) * 2 * T);  // + the zero slot, never written
+    G.cs8 = (uint32_t*)P(N * kCellSlots * 4);
     G.cs_off = (uint32_t*)P((N + 1) * 4);
     G.gwl = (double*)P(N * 16);
     G.cs_idx = (uint32_t*)P(D.cs_count * 4);
@@ -663,6 +671,7 @@ int commit_design(Ctx* c, Prepared& P, uint32_t* id_out) {
     if (!rc) rc = copy_in(c, D.dev.hf_pins, L.hf_pins.data(), L.hf_pins.size() * 4);
     if (!rc) rc = copy_in(c, D.dev.cs_off, L.cs_off.data(), L.cs_off.size() * 4);
     if (!rc) rc = copy_in(c, D.dev.cs_idx, L.cs_idx.data(), L.cs_idx.size() * 4);
+    if (!rc) rc = copy_in(c, D.dev.cs8, L.cs8.data(), L.cs8.size() * 4);
     if (!rc) rc = copy_in(c, D.dev.chf_off, L.chf_off.data(), L.chf_off.size() * 4);
     if (!rc) rc = copy_in(c, D.dev.chf_net, L.chf_net.data(), L.chf_net.size() * 4);
     if (!rc && D.dev.slot_w) rc = copy_in(c, (void*)D.dev.slot_w, P.slot_w.data(), P.slot_w.size());

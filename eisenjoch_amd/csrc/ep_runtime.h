@@ -124,6 +124,7 @@ struct NetLayout {
     int bucket_group[kMaxBuckets] = {0};
     std::vector<uint32_t> hf_off, hf_pins;
     std::vector<uint32_t> cs_off, cs_idx;
+    std::vector<uint32_t> cs8;      // [cell][kCellSlots]: first slots of each cell, then the zero slot
     std::vector<int32_t> slot_net;  // net of each slot of `slots` (-1: padding)
     std::vector<int32_t> hf_net;    // net of each high-fanout net
     std::vector<uint32_t> chf_off, chf_net;  // cell -> high-fanout nets it has pins on
