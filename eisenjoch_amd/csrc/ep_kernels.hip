@@ -144,6 +144,51 @@ __device__ inline int wa_group(const DesignDev& D, int w) {
     return D.bucket_group[bk];
 }
 
+// Butterfly partner of a group reduction, by DPP where the hardware offers
+// it: quad_perm for distances 1 and 2; for 4 and 8 the half-row and row
+// mirrors, which pair each lane with a lane of the other half-group (the
+// earlier steps left every lane of a half-group the same value, so the
+// result equals the xor partner's bit for bit).  Distances 16 and 32 use
+// the LDS crossbar.
+template <int O>
+__device__ inline int dpp_partner(int v) {
+    if constexpr (O == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
+    else if constexpr (O == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    else if constexpr (O == 4) return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true);  // row_half_mirror
+    else return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, true);                      // row_mirror
+}
+template <int O>
+__device__ inline float partner(float v) {
+    if constexpr (O <= 8) return __int_as_float(dpp_partner<O>(__float_as_int(v)));
+    else return __shfl_xor(v, O, kWave);
+}
+template <int O>
+__device__ inline double partner(double v) {
+    if constexpr (O <= 8) {
+        const int lo = dpp_partner<O>(__double2loint(v)), hi = dpp_partner<O>(__double2hiint(v));
+        return __hiloint2double(hi, lo);
+    } else {
+        return __shfl_xor(v, O, kWave);
+    }
+}
+
+// One reduction step of distance O over groups of G lanes (no-op for G <= O).
+template <int O, typename T>
+__device__ inline void group_step(int G, T& xmx, T& xmn, T& ymx, T& ymn) {
+    if (G > O) {
+        xmx = fmax(xmx, partner<O>(xmx));
+        xmn = fmin(xmn, partner<O>(xmn));
+        ymx = fmax(ymx, partner<O>(ymx));
+        ymn = fmin(ymn, partner<O>(ymn));
+    }
+}
+template <int O, typename T>
+__device__ inline void group_sum(int G, T (&s)[8]) {
+    if (G > O)
+#pragma unroll
+        for (int q = 0; q < 8; q++) s[q] += partner<O>(s[q]);
+}
+
 // One 64-slot unit w of the net layout (nets of G pins): the group
 // reductions of its nets.  HPWL mode: h (lane 0) is the unit's partial of sum
 // over nets of (max - min) x + y of the solver positions.  Else (gx, gy) is
@@ -158,12 +203,12 @@ __device__ inline void wa_unit(const WaArgs<T>& A, int G, int lane, uint32_t c, 
     const T big = Lim<T>::big();
     T xmx = valid ? px : -big, xmn = valid ? px : big;
     T ymx = valid ? py : -big, ymn = valid ? py : big;
-    for (int o = 1; o < G; o <<= 1) {
-        xmx = fmax(xmx, shfl_xor(xmx, o));
-        xmn = fmin(xmn, shfl_xor(xmn, o));
-        ymx = fmax(ymx, shfl_xor(ymx, o));
-        ymn = fmin(ymn, shfl_xor(ymn, o));
-    }
+    group_step<1>(G, xmx, xmn, ymx, ymn);
+    group_step<2>(G, xmx, xmn, ymx, ymn);
+    group_step<4>(G, xmx, xmn, ymx, ymn);
+    group_step<8>(G, xmx, xmn, ymx, ymn);
+    group_step<16>(G, xmx, xmn, ymx, ymn);
+    group_step<32>(G, xmx, xmn, ymx, ymn);
     if (HPWL) {
         // wave padding forms whole groups without a valid pin (max < min): skip them
         const bool has = xmx >= xmn;
@@ -176,9 +221,12 @@ __device__ inline void wa_unit(const WaArgs<T>& A, int G, int lane, uint32_t c, 
     const T ex = valid ? fexp(a * ux) : T(0), fx = valid ? fexp(-a * vx) : T(0);
     const T ey = valid ? fexp(a * uy) : T(0), fy = valid ? fexp(-a * vy) : T(0);
     T s[8] = {ex, ux * ex, fx, vx * fx, ey, uy * ey, fy, vy * fy};
-    for (int o = 1; o < G; o <<= 1)
-#pragma unroll
-        for (int q = 0; q < 8; q++) s[q] += shfl_xor(s[q], o);
+    group_sum<1>(G, s);
+    group_sum<2>(G, s);
+    group_sum<4>(G, s);
+    group_sum<8>(G, s);
+    group_sum<16>(G, s);
+    group_sum<32>(G, s);
     gx = wa_pin<T>(a, ux, ex, s[0], s[1], vx, fx, s[2], s[3]);
     gy = wa_pin<T>(a, uy, ey, s[4], s[5], vy, fy, s[6], s[7]);
 }
