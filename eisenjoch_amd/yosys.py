@@ -11,7 +11,7 @@ objects into serde_json maps without `preserve_order` (`Cargo.toml`:
 order is used here, which fixes the order in which cells and nets are
 created and which module is "first".
 
-`to_flat_design` is this build's own bridge from a parsed netlist to the
+`to_flat_design` is this package's own bridge from a parsed netlist to the
 placer: a Yosys netlist carries no device geometry, so the grid, the IO
 locations and the initial positions are chosen here (see its docstring).
 """
