@@ -26,6 +26,7 @@ from .placer import (  # noqa: F401
     place_electro,
     sized_design,
     total_hpwl_int,
+    with_spacers,
 )
 
 __version__ = "0.1.0"

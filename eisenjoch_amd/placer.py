@@ -273,6 +273,41 @@ def bins_for_side(side: int) -> int:
     return int(min(512, max(64, b)))
 
 
+def with_spacers(d: FlatDesign, target_util: float = 0.7, spacer_area: float = 1.0,
+                 seed: int = 0):
+    """Spacer (filler) cells up to `target_util` of the grid, the reference
+    design spec's section 1f (docs/superpowers/specs/2026-03-13-placer-
+    optimization-design.md:91-97), which `ElectroPlaceCfg.target_util`
+    (config.rs) names but the shipped placer never inserts:
+
+        spacer_count = (grid area * target_util - movable area) / spacer_area
+
+    Spacers are movable cells of `spacer_area` without pins (they take part in
+    the density, not in the wirelength), appended after the design's movable
+    cells at seeded uniform positions; the fixed cells' pin indices move up by
+    their number.  The first `d.n_mov` positions of a placement are the
+    design's own cells.  Returns (design with spacers, spacer count)."""
+    area = d.area.astype(np.float64) if d.area is not None else np.ones(d.n_mov)
+    n_sp = int(math.floor((d.W * d.H * float(target_util) - float(area.sum())) / float(spacer_area)))
+    if n_sp <= 0:
+        return d, 0
+    rng = np.random.default_rng(seed)
+    pins = d.pin_cell.astype(np.int64)
+    pins = np.where(pins >= d.n_mov, pins + n_sp, pins).astype(np.uint32)
+    # a spacer's footprint is its area's square; keep it inside the grid
+    half = 0.5 * math.sqrt(spacer_area)
+    sx = rng.uniform(half, max(half, d.W - half), n_sp)
+    sy = rng.uniform(half, max(half, d.H - half), n_sp)
+    out = FlatDesign(
+        W=d.W, H=d.H, Bx=d.Bx, By=d.By,
+        x0=np.concatenate([d.x0, sx]), y0=np.concatenate([d.y0, sy]),
+        fixed_x=d.fixed_x, fixed_y=d.fixed_y, net_off=d.net_off, pin_cell=pins,
+        area=np.concatenate([area, np.full(n_sp, spacer_area)]).astype(np.float32),
+        pinw=None if d.pinw is None else np.concatenate([d.pinw, np.zeros(n_sp)]),
+        net_weight=d.net_weight, name=d.name + "+spacers")
+    return out, n_sp
+
+
 def sized_design(n_cells: int, seed: int, name: str = "") -> FlatDesign:
     """Batch/serving rule: side = round(sqrt(N / 0.7)), nets = 1.1 N,
     IO = perimeter / 3, clusters of ~200 cells, areas 1/2/4."""
