@@ -25,6 +25,7 @@ from .placer import (  # noqa: F401
     legalize,
     place_electro,
     sized_design,
+    timing_net_weights,
     total_hpwl_int,
     with_spacers,
 )

@@ -308,6 +308,21 @@ def with_spacers(d: FlatDesign, target_util: float = 0.7, spacer_area: float = 1
     return out, n_sp
 
 
+def timing_net_weights(crit, timing_weight: float = 5.0) -> np.ndarray:
+    """Net weights of timing-driven placement, 1 + timing_weight * crit^2 per
+    net (the reference design spec's section 1i,
+    docs/superpowers/specs/2026-03-13-placer-optimization-design.md:109-113;
+    `ElectroPlaceCfg.timing_driven` / `timing_weight` in config.rs name it,
+    the shipped placer never applies it).  `crit` holds each net's
+    criticality in [0, 1] from the caller's timing analysis; the result is
+    `FlatDesign.net_weight`, which the WA gradient multiplies into every pin
+    of the net (common.rs:246-261)."""
+    c = np.asarray(crit, dtype=np.float64)
+    if c.size and (not np.all(np.isfinite(c)) or c.min() < 0.0 or c.max() > 1.0):
+        raise ValueError("criticalities must lie in [0, 1]")
+    return 1.0 + float(timing_weight) * c * c
+
+
 def sized_design(n_cells: int, seed: int, name: str = "") -> FlatDesign:
     """Batch/serving rule: side = round(sqrt(N / 0.7)), nets = 1.1 N,
     IO = perimeter / 3, clusters of ~200 cells, areas 1/2/4."""

@@ -1,7 +1,8 @@
-"""Spacer (filler) insertion to the target utilisation (the reference design
-spec, docs/superpowers/specs/2026-03-13-placer-optimization-design.md:91-97:
-spacer_count = (grid area x 0.7 - movable area) / spacer area; spacers take
-part in the density, not the wirelength)."""
+"""Spacer (filler) insertion to the target utilisation and timing-driven net
+weights, from the reference design spec
+(docs/superpowers/specs/2026-03-13-placer-optimization-design.md): :91-97,
+spacer_count = (grid area x 0.7 - movable area) / spacer area, spacers take
+part in the density, not the wirelength; :109-113, net weight 1 + 5 crit^2."""
 import math
 
 import numpy as np
@@ -26,6 +27,18 @@ def test_spacer_count_area_and_pin_remap():
     assert np.array_equal(s.net_off, d.net_off)
     sx, sy = s.x0[d.n_mov:], s.y0[d.n_mov:]
     assert sx.min() >= 0.5 and sx.max() <= d.W - 0.5 and sy.min() >= 0.5 and sy.max() <= d.H - 0.5
+
+
+def test_timing_net_weights():
+    w = ep.timing_net_weights([0.0, 0.5, 1.0])
+    assert np.allclose(w, [1.0, 2.25, 6.0])
+    assert np.allclose(ep.timing_net_weights([1.0], timing_weight=2.0), [3.0])
+    with pytest.raises(ValueError):
+        ep.timing_net_weights([1.5])
+    # the weights are the design's per-net weights
+    d = ep.config1()
+    d.net_weight = ep.timing_net_weights(np.linspace(0.0, 1.0, d.n_nets))
+    assert d.to_c().net_weight
 
 
 def test_no_spacers_when_full():
