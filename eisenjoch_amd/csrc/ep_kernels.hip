@@ -67,14 +67,50 @@ __device__ inline int design_of(const int32_t* tab, const int32_t* pfx, int nd, 
 template <typename T>
 __device__ inline T shfl_xor(T v, int o) { return __shfl_xor(v, o, kWave); }
 
+// Butterfly partner of a group reduction, by DPP where the hardware offers
+// it: quad_perm for distances 1 and 2; for 4 and 8 the half-row and row
+// mirrors, which pair each lane with a lane of the other half-group (the
+// earlier steps left every lane of a half-group the same value, so the
+// result equals the xor partner's bit for bit).  Distances 16 and 32 use
+// the LDS crossbar.
+template <int O>
+__device__ inline int dpp_partner(int v) {
+    if constexpr (O == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
+    else if constexpr (O == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    else if constexpr (O == 4) return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true);  // row_half_mirror
+    else return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, true);                      // row_mirror
+}
+template <int O>
+__device__ inline float partner(float v) {
+    if constexpr (O <= 8) return __int_as_float(dpp_partner<O>(__float_as_int(v)));
+    else return __shfl_xor(v, O, kWave);
+}
+template <int O>
+__device__ inline double partner(double v) {
+    if constexpr (O <= 8) {
+        const int lo = dpp_partner<O>(__double2loint(v)), hi = dpp_partner<O>(__double2hiint(v));
+        return __hiloint2double(hi, lo);
+    } else {
+        return __shfl_xor(v, O, kWave);
+    }
+}
+
+
 // Deterministic block sum of NV doubles (256 threads); result in lane 0 of wave 0.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double* sh) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; q++) {
+        // ascending butterfly: distances 1-8 by DPP lane moves, 16 and 32
+        // through the LDS crossbar; every lane ends with the wave's sum
         double x = v[q];
-        for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, kWave);
+        x += partner<1>(x);
+        x += partner<2>(x);
+        x += partner<4>(x);
+        x += partner<8>(x);
+        x += partner<16>(x);
+        x += partner<32>(x);
         v[q] = x;
     }
     if (lane == 0)
@@ -142,34 +178,6 @@ __device__ inline int wa_group(const DesignDev& D, int w) {
 #pragma unroll
     for (int q = 1; q < kMaxBuckets; q++) bk += (q < D.n_buckets && w >= D.bucket_wave_start[q]) ? 1 : 0;
     return D.bucket_group[bk];
-}
-
-// Butterfly partner of a group reduction, by DPP where the hardware offers
-// it: quad_perm for distances 1 and 2; for 4 and 8 the half-row and row
-// mirrors, which pair each lane with a lane of the other half-group (the
-// earlier steps left every lane of a half-group the same value, so the
-// result equals the xor partner's bit for bit).  Distances 16 and 32 use
-// the LDS crossbar.
-template <int O>
-__device__ inline int dpp_partner(int v) {
-    if constexpr (O == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
-    else if constexpr (O == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
-    else if constexpr (O == 4) return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true);  // row_half_mirror
-    else return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, true);                      // row_mirror
-}
-template <int O>
-__device__ inline float partner(float v) {
-    if constexpr (O <= 8) return __int_as_float(dpp_partner<O>(__float_as_int(v)));
-    else return __shfl_xor(v, O, kWave);
-}
-template <int O>
-__device__ inline double partner(double v) {
-    if constexpr (O <= 8) {
-        const int lo = dpp_partner<O>(__double2loint(v)), hi = dpp_partner<O>(__double2hiint(v));
-        return __hiloint2double(hi, lo);
-    } else {
-        return __shfl_xor(v, O, kWave);
-    }
 }
 
 // One reduction step of distance O over groups of G lanes (no-op for G <= O).
