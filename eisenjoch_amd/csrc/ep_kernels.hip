@@ -833,24 +833,32 @@ __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy)
     const uint32_t e = D.cs_off[i + 1];
     const uint32_t h0 = D.chf_off[i], h1 = D.chf_off[i + 1];
     {
-        // the first eight slots (most cells have no more) come from the
-        // fixed-stride copy, loaded with the bounds: their values are the
-        // second round of loads.  Past the cell's last pin the row names the
-        // zero slot, whose +0.0 leaves the sum unchanged bit for bit (it
-        // starts at +0.0 and never becomes -0.0).
-        static_assert(kCellSlots == 8, "two 16-byte loads per cell");
-        const uint4 a = ((const uint4*)D.cs8)[2 * (size_t)i];
-        const uint4 c = ((const uint4*)D.cs8)[2 * (size_t)i + 1];
-        const uint32_t s8[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
-        V v8[8];
+        // the first kCellSlots slots (all but a few cells in a thousand have
+        // no more; with eight, nearly every wave held a cell with more) come
+        // from the fixed-stride copy, loaded with the bounds: their values
+        // are the second round of loads.  Past the cell's last pin the row
+        // names the zero slot, whose +0.0 leaves the sum unchanged bit for
+        // bit (it starts at +0.0 and never becomes -0.0).
+        static_assert(kCellSlots % 4 == 0, "16-byte loads of the fixed-stride row");
+        constexpr int kQ = kCellSlots / 4;
+        uint32_t sl[kCellSlots];
 #pragma unroll
-        for (int q = 0; q < 8; q++) v8[q] = pg[s8[q]];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            sx += (double)v8[q].x;
-            sy += (double)v8[q].y;
+        for (int q = 0; q < kQ; q++) {
+            const uint4 a = ((const uint4*)D.cs8)[kQ * (size_t)i + q];
+            sl[4 * q] = a.x;
+            sl[4 * q + 1] = a.y;
+            sl[4 * q + 2] = a.z;
+            sl[4 * q + 3] = a.w;
         }
-        k += min(e - k, 8u);
+        V vs[kCellSlots];
+#pragma unroll
+        for (int q = 0; q < kCellSlots; q++) vs[q] = pg[sl[q]];
+#pragma unroll
+        for (int q = 0; q < kCellSlots; q++) {
+            sx += (double)vs[q].x;
+            sy += (double)vs[q].y;
+        }
+        k += min(e - k, (uint32_t)kCellSlots);
     }
     for (; k + 4 <= e; k += 4) {
         const uint32_t s0 = D.cs_idx[k], s1 = D.cs_idx[k + 1];
