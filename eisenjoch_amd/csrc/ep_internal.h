@@ -95,7 +95,7 @@ struct DesignDev {
     void* pg;                  // per-slot WA gradient (gx, gy) in T [n_slots + 1][2]; entry n_slots
                                // is never written and stays zero
     uint32_t* cs_off;          // cell -> slot CSR [n_mov + 1]
-    double* gwl;               // [n_mov][2] WA gradient per cell (its slots summed in FP64)
+    void* gwl;                 // T [n_mov][2] WA gradient per cell (its slots summed in FP64, stored in T)
     const void* slot_w;        // T [n_slots]: each slot's net weight, or null (all nets weigh 1)
     uint32_t* cs_idx;          // slots of each movable cell, ascending (nets of <= 64 pins)
     uint32_t* cs8;             // [n_mov][kCellSlots]: the first kCellSlots of them, then slot n_slots

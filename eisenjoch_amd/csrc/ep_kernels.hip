@@ -912,7 +912,10 @@ __global__ __launch_bounds__(256) void k_wl_cell(BatchDev b) {
     if (i >= D.n_mov) return;
     double2 v;
     gwl_of<T>(D, i, v.x, v.y);
-    ((double2*)D.gwl)[i] = v;
+    typename Vec2<T>::type w;  // summed in FP64, kept in the arithmetic type
+    w.x = (T)v.x;
+    w.y = (T)v.y;
+    ((typename Vec2<T>::type*)D.gwl)[i] = w;
 }
 
 // MODE 0: gather; MODE 1: fix (penalty set by ctrl this iteration);
@@ -936,7 +939,8 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
         double2 wl = make_double2(0.0, 0.0);
         double pw = 0.0, dpv[2] = {0.0, 0.0}, gpv[2] = {0.0, 0.0};
         if (MODE == 0) {
-            wl = ((const double2*)D.gwl)[i];
+            const typename Vec2<T>::type w = ((const typename Vec2<T>::type*)D.gwl)[i];
+            wl = make_double2((double)w.x, (double)w.y);
             pw = (double)((const T*)D.pinw)[i];
 #pragma unroll
             for (int ax = 0; ax < 2; ax++) {
@@ -1011,7 +1015,8 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
         } else {
             const double lam = MODE == 0 ? sched(D.cfg, st->lam) : st->lam;
             if (MODE != 0) {  // the fix pass reads its operands here
-                wl = ((const double2*)D.gwl)[i];
+                const typename Vec2<T>::type w = ((const typename Vec2<T>::type*)D.gwl)[i];
+                wl = make_double2((double)w.x, (double)w.y);
                 pw = (double)((const T*)D.pinw)[i];
             }
             double pc = pw + lam;  // mod.rs:152-156

@@ -291,7 +291,7 @@ static size_t plan_arena(Design& D, char* base) {
     G.pg = P(((size_t)G.n_slots + 1) * 2 * T);  // + the zero slot, never written
     G.cs8 = (uint32_t*)P(N * kCellSlots * 4);
     G.cs_off = (uint32_t*)P((N + 1) * 4);
-    G.gwl = (double*)P(N * 16);
+    G.gwl = P(N * 2 * T);
     G.cs_idx = (uint32_t*)P(D.cs_count * 4);
     G.chf_off = (uint32_t*)P((N + 1) * 4);
     G.chf_net = (uint32_t*)P(D.chf_count * 4);
