@@ -329,6 +329,26 @@ def test_batch_equals_alone_bitwise(device):
             device.free(i)
 
 
+def test_batch_done_times_follow_iterations(device):
+    # a batch advances its designs in lockstep, so the device-clock time from
+    # the reset to a design's stop grows with the iteration it stopped at
+    ds = [ep.sized_design(n, 200 + k) for k, n in enumerate([2000, 5000, 12000, 30000])]
+    cfg = ep.ElectroPlaceCfg(max_iters=300, field_mode="idxst", log_hpwl=False, stop_every_iter=True)
+    log_case("batch done times")
+    ids = [device.upload(d, "fast") for d in ds]
+    try:
+        rs = device.place_batch(ids, cfg, positions=False)
+        t = np.array([r.t_done_ms for r in rs])
+        it = np.array([r.iters for r in rs])
+        assert np.all(np.isfinite(t)) and np.all(t > 0.0)
+        order = np.argsort(it, kind="stable")
+        assert np.all(np.diff(t[order]) >= -1e-3 * t.max() - 1e-3)  # non-decreasing in iterations
+        assert np.all(np.array([r.t_submit_ms for r in rs]) == 0.0)
+    finally:
+        for i in ids:
+            device.free(i)
+
+
 def test_graph_replay_equals_eager(device):
     d = ep.sized_design(6000, 7)
     cfg = ep.ElectroPlaceCfg(max_iters=120, field_mode="idxst")
