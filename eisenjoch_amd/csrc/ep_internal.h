@@ -25,7 +25,7 @@ constexpr int kScatBins = 6144;      // LDS accumulation window of a scatter blo
 constexpr int kHfStat = 16;      // per high-fanout net: 4 extremes, 8 sums, weight, pad
 constexpr int kXcds = 8;          // MI355X: blocks are dealt to the 8 XCDs in turn
 constexpr int kCellSlots = 12;    // slots of a cell held at a fixed stride (cs8)
-constexpr int kWaUnits = 2;       // 64-slot net units per wave of the WA kernel
+constexpr int kWaUnits = 3;       // 64-slot net units per wave of the WA kernel
 // blocks (four waves) of the WA kernel for a design of n_wa_waves units
 inline int wa_blocks_of(int n_wa_waves) { return (n_wa_waves + 4 * kWaUnits - 1) / (4 * kWaUnits); }
 constexpr int kSplitMinK = 512;   // K from which transform products are split
