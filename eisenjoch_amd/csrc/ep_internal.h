@@ -119,7 +119,9 @@ struct DesignDev {
     uint32_t* hf_off;          // [n_hf + 1]  (nets with > 64 pins)
     uint32_t* hf_pins;
     // bins
-    void* dacc;                // FAST: u64 fixed point; FP64: double [By*Bx]
+    void* dacc;                // wide-cell scatter accumulator: FAST u64 fixed point 2^-32 [By*Bx];
+                               // FP64 exact 128-bit fixed point, integer words [By*Bx] then
+                               // 2^-64 fraction words [By*Bx]
     void* dens;                // T [By*Bx]
     // transform intermediates; a pass whose products are split along K
     // writes ks planes of partial sums (ks_x for K = Bx, ks_y for K = By)

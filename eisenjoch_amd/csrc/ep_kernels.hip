@@ -445,6 +445,29 @@ __device__ inline BinBox bin_box(const DesignDev& D, double cx, double cy, doubl
     return r;
 }
 
+// Density accumulators.  FAST: 64-bit fixed point 2^-32 (one word per bin).
+// FP64: exact 128-bit fixed point, an integer word and a 2^-64 fraction word
+// per bin (FIX = false); the fraction's carry goes to the integer word, so the
+// sum is the exact sum of the truncated contributions in any order and the
+// plain placer's density is deterministic to the last bit.
+template <bool FIX>
+__device__ inline void acc_add(unsigned long long* whole, unsigned long long* frac, double w) {
+    if constexpr (FIX) {
+        atomicAdd(whole, (unsigned long long)__double2ll_rn(w * kFixScale));
+    } else {
+        const double ip = floor(w);
+        const unsigned long long f = __double2ull_rz((w - ip) * 18446744073709551616.0);
+        const unsigned long long old = atomicAdd(frac, f);
+        const unsigned long long carry = (unsigned long long)ip + ((old + f < old) ? 1ull : 0ull);
+        if (carry) atomicAdd(whole, carry);
+    }
+}
+template <bool FIX>
+__device__ inline double acc_value(unsigned long long whole, unsigned long long frac) {
+    if constexpr (FIX) return (double)(long long)whole * (1.0 / kFixScale);
+    else return (double)whole + (double)frac * 5.421010862427522e-20;  // 2^-64
+}
+
 // density.rs:41-48
 __device__ inline double ov_area(const BinBox& r, int bx, int by) {
     double ox = fmin(r.xh, (double)bx + 1.0) - fmax(r.xl, (double)bx);
@@ -464,7 +487,9 @@ __device__ inline double ov_area(const BinBox& r, int bx, int by) {
 // path changes the result.
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
-    typedef typename std::conditional<FIX, unsigned long long, double>::type Acc;
+    typedef unsigned long long Acc;
+    // FAST: kScatBins one-word bins; FP64: half as many two-word bins
+    constexpr int kCap = FIX ? kScatBins : kScatBins / 2;
     __shared__ Acc win[kScatBins];
     __shared__ int box[4];
     const int d = design_of(b.blk_scat, b.pfx_scat, b.nd, blockIdx.x);
@@ -474,6 +499,9 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
     const uint32_t* PERM = D.perm[(D.st->it + 1) & 1];
     typedef typename Vec2<T>::type V;
     const V* LXY = (const V*)D.look;
+    const size_t nb = (size_t)D.Bx * D.By;
+    Acc* GW = (Acc*)D.dacc;       // integer / fixed-point words
+    Acc* GF = GW + nb;            // FP64 fraction words
     if (threadIdx.x == 0) {
         box[0] = box[2] = 0x7fffffff;
         box[1] = box[3] = -1;
@@ -495,9 +523,9 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
     __syncthreads();
     const int bx0 = box[0], by0 = box[2];
     const int ww = box[1] - box[0], hh = box[3] - box[2];
-    const bool local = ww > 0 && hh > 0 && (long long)ww * hh <= kScatBins;
+    const bool local = ww > 0 && hh > 0 && (long long)ww * hh <= kCap;
     if (local)
-        for (int k = threadIdx.x; k < ww * hh; k += blockDim.x) win[k] = Acc(0);
+        for (int k = threadIdx.x; k < (FIX ? 1 : 2) * ww * hh; k += blockDim.x) win[k] = Acc(0);
     __syncthreads();
     for (int k = threadIdx.x; k < kScatCells; k += blockDim.x) {
         if (c0 + k >= D.n_mov) break;
@@ -507,19 +535,32 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
         for (int by = r.by0; by < r.by1; by++)
             for (int bx = r.bx0; bx < r.bx1; bx++) {
                 const double w = ov_area(r, bx, by);
-                Acc v;
-                if constexpr (FIX) v = (Acc)__double2ll_rn(w * kFixScale);
-                else v = w;
-                if (local) atomicAdd(&win[(by - by0) * ww + (bx - bx0)], v);
-                else atomicAdd(&((Acc*)D.dacc)[(size_t)by * D.Bx + bx], v);
+                if (local) {
+                    const int q = (by - by0) * ww + (bx - bx0);
+                    acc_add<FIX>(&win[q], &win[ww * hh + q], w);
+                } else {
+                    const size_t q = (size_t)by * D.Bx + bx;
+                    acc_add<FIX>(&GW[q], &GF[q], w);
+                }
             }
     }
     if (!local) return;
     __syncthreads();
     for (int k = threadIdx.x; k < ww * hh; k += blockDim.x) {
+        const size_t q = (size_t)(by0 + k / ww) * D.Bx + bx0 + k % ww;
         const Acc v = win[k];
-        if (v != Acc(0))
-            atomicAdd(&((Acc*)D.dacc)[(size_t)(by0 + k / ww) * D.Bx + bx0 + k % ww], v);
+        if (FIX) {
+            if (v != Acc(0)) atomicAdd(&GW[q], v);
+        } else {
+            // the window's fraction word goes in with its carry, the whole word after
+            const Acc f = win[ww * hh + k];
+            Acc carry = v;
+            if (f != Acc(0)) {
+                const Acc old = atomicAdd(&GF[q], f);
+                carry += (old + f < old) ? 1ull : 0ull;
+            }
+            if (carry) atomicAdd(&GW[q], carry);
+        }
     }
 }
 
@@ -534,15 +575,14 @@ __global__ __launch_bounds__(256) void k_dens_final(BatchDev b) {
     const int nb = D.Bx * D.By;
     double v = 0.0;
     if (i < nb) {
+        unsigned long long* a = (unsigned long long*)D.dacc;
         if constexpr (FIX) {
-            unsigned long long* a = (unsigned long long*)D.dacc;
-            v = (double)(long long)a[i] * (1.0 / kFixScale);
-            a[i] = 0ull;
+            v = acc_value<true>(a[i], 0ull);
         } else {
-            double* a = (double*)D.dacc;
-            v = a[i];
-            a[i] = 0.0;
+            v = acc_value<false>(a[i], a[(size_t)nb + i]);
+            a[(size_t)nb + i] = 0ull;
         }
+        a[i] = 0ull;
         ((T*)D.dens)[i] = (T)v;
     }
     double p[kBinParts] = {v, v > 1.0 ? v - 1.0 : 0.0};
@@ -728,8 +768,9 @@ __global__ __launch_bounds__(256) void k_tile_fill(BatchDev b) {
 // overflow partials with plain stores: no global atomics.
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
-    typedef typename std::conditional<FIX, unsigned long long, double>::type Acc;
-    __shared__ Acc win[32 * 32];
+    typedef unsigned long long Acc;
+    // FAST: one fixed-point word per bin; FP64: the whole words, then the fractions
+    __shared__ Acc win[(FIX ? 1 : 2) * 32 * 32];
     __shared__ double sh[4 * kBinParts];
     const int d = design_of(b.blk_tile, b.pfx_tile, b.nd, blockIdx.x);
     const DesignDev& D = b.d[d];
@@ -768,7 +809,7 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
         rk0[q] = k0;
         rlen[q] = k1 - k0;
     }
-    for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) win[k] = Acc(0);
+    for (int k = threadIdx.x; k < (FIX ? 1 : 2) * 32 * 32; k += blockDim.x) win[k] = Acc(0);
     __syncthreads();
     uint32_t pre[11];  // start of range q in the joint index space
     pre[0] = 0;
@@ -788,10 +829,8 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
         for (int by = by0; by < by1; by++)
             for (int bx = bx0; bx < bx1; bx++) {
                 const double w = ov_area(r, bx, by);
-                Acc v;
-                if constexpr (FIX) v = (Acc)__double2ll_rn(w * kFixScale);
-                else v = w;
-                atomicAdd(&win[(by - y0) * 32 + (bx - x0)], v);
+                const int q = (by - y0) * 32 + (bx - x0);
+                acc_add<FIX>(&win[q], &win[FIX ? q : 32 * 32 + q], w);
             }
     }
     __syncthreads();
@@ -799,9 +838,7 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
     for (int k = threadIdx.x; k < 32 * 32; k += blockDim.x) {
         const int bx = x0 + (k & 31), by = y0 + (k >> 5);
         if (bx < D.Bx && by < D.By) {
-            double v;
-            if constexpr (FIX) v = (double)(long long)win[k] * (1.0 / kFixScale);
-            else v = win[k];
+            const double v = acc_value<FIX>(win[k], FIX ? 0ull : win[32 * 32 + k]);
             ((T*)D.dens)[(size_t)by * D.Bx + bx] = (T)v;
             p[0] += v;
             p[1] += v > 1.0 ? v - 1.0 : 0.0;

@@ -313,7 +313,7 @@ static size_t plan_arena(Design& D, char* base) {
     G.slots = (uint32_t*)P((size_t)G.n_wa_waves * kWave * 4);
     G.hf_off = (uint32_t*)P((size_t)(G.n_hf + 1) * 4);
     G.hf_pins = (uint32_t*)P(D.hf_pin_count * 4);
-    G.dacc = P(B * 8);
+    G.dacc = P(B * (D.prec == EP_PREC_FP64 ? 16 : 8));  // FP64: integer and fraction words
     G.dens = P(B * T);
     G.tT = P(B * T * G.ks_x);
     for (int ax = 0; ax < 2; ax++) G.ehat[ax] = P(B * T * G.ks_y);
