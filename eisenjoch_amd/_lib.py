@@ -114,6 +114,7 @@ SIGNATURES = {
     "ep_design_free": (C.c_int, [_p, C.c_uint32]),
     "ep_design_bytes": (C.c_uint64, [C.POINTER(EpDesign), C.c_int32]),
     "ep_place": (C.c_int, [_p, C.c_uint32, C.POINTER(EpConfig), C.POINTER(EpResult)]),
+    "ep_place_noalloc": (C.c_int, [_p, C.c_uint32, C.POINTER(EpConfig), C.POINTER(EpResult)]),
     "ep_place_batch": (C.c_int, [_p, C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(EpConfig),
                                  C.POINTER(EpResult)]),
     "ep_positions": (C.c_int, [_p, C.c_uint32, _p, _p]),

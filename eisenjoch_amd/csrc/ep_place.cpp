@@ -64,9 +64,12 @@ int engine_for(Ctx* c, int prec, int Bx, int By) {
     return it == c->tune.end() ? 0 : it->second;
 }
 
-// Per pass: (A kind, A matrix or buffer, B kind, B matrix or buffer, epilogue, out, out2)
+// Per pass: (A kind, A matrix or buffer, B kind, B matrix or buffer, epilogue, out, out2).
+// The device image (descriptors, prefixes, jobs, block tables) goes into a
+// fresh allocation, or, with ext_mem, into that caller-owned device buffer of
+// ext_cap bytes (EP_ERR_INVALID_ARG when it does not fit; nothing is allocated).
 int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<const ep_config*>& cfgs,
-                     Batch& b) {
+                     Batch& b, void* ext_mem, size_t ext_cap) {
     batch_release(b);
     b.ds = ds;
     const int nd = (int)ds.size();
@@ -259,7 +262,17 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             std::fill(blk[g].begin() + P[i], blk[g].begin() + P[i + 1], i);
         bytes += align_up(sizeof(int32_t) * blk[g].size());
     }
-    HIPCHK(c, hipMalloc(&b.mem, bytes));
+    if (ext_mem) {
+        if (bytes > ext_cap)
+            return fail(c, EP_ERR_INVALID_ARG,
+                        "launch descriptors of this batch need " + std::to_string(bytes) +
+                            " bytes; the preallocated device buffer holds " + std::to_string(ext_cap));
+        b.mem = ext_mem;
+        b.ext_mem = true;
+    } else {
+        HIPCHK(c, hipMalloc(&b.mem, bytes));
+        b.ext_mem = false;
+    }
     std::vector<char> img(bytes, 0);
     size_t off = 0;
     auto put = [&](const void* src, size_t n) {
@@ -301,7 +314,13 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         b.sum_blocks = 0;
         for (auto& ps : sums) b.sum_blocks = std::max(b.sum_blocks, ps.blocks);
     }
-    HIPCHK(c, hipMemcpy(b.mem, img.data(), bytes, hipMemcpyHostToDevice));
+    if (b.ext_mem) {
+        // stream-ordered behind any earlier use of the buffer on this stream
+        HIPCHK(c, hipMemcpyAsync(b.mem, img.data(), bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));  // img is read until here
+    } else {
+        HIPCHK(c, hipMemcpy(b.mem, img.data(), bytes, hipMemcpyHostToDevice));
+    }
     b.cfgs.assign(nd, ep_config{});
     for (int i = 0; i < nd; i++) b.cfgs[i] = *cfgs[i];
     return EP_OK;
@@ -309,7 +328,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
 
 int batch_build(Ctx* c, const std::vector<Design*>& ds, const ep_config* cfg, Batch& b) {
     std::vector<const ep_config*> cf(ds.size(), cfg);
-    return batch_build_cfgs(c, ds, cf, b);
+    return batch_build_cfgs(c, ds, cf, b, nullptr, 0);
 }
 
 void batch_release(Batch& b) {
@@ -318,8 +337,9 @@ void batch_release(Batch& b) {
         b.graph[k] = nullptr;
         b.graph_iters[k] = 0;
     }
-    if (b.mem) hipFree(b.mem);
+    if (b.mem && !b.ext_mem) hipFree(b.mem);
     b.mem = nullptr;
+    b.ext_mem = false;
     b.ds.clear();
 }
 
@@ -720,7 +740,7 @@ static int time_transforms(Ctx* c, const std::vector<Design*>& ds, uint32_t reps
     release_batch_cache(c);
     Batch b;
     std::vector<const ep_config*> cf(ds.size(), &cfg);
-    int rc = batch_build_cfgs(c, ds, cf, b);
+    int rc = batch_build_cfgs(c, ds, cf, b, nullptr, 0);
     if (rc) return rc;
     // live design states so the transform kernels do not skip them
     for (Design* D : ds) {

@@ -448,6 +448,10 @@ int ep_create(ep_ctx** out, int device, uint64_t budget) {
     for (int i = kStatePool - 1; i >= 0; i--) c->free_states.push_back(i);
     ep_config_default(&c->eval_cfg);
     c->t0 = now_ms();
+    {
+        int rc = noalloc_init(c.get());  // this device's ep_place_noalloc buffer
+        if (rc) return rc;
+    }
     *out = (ep_ctx*)c.release();
     return EP_OK;
 }
@@ -508,6 +512,18 @@ int ep_design_upload(ep_ctx* h, const ep_design* d, int32_t prec, uint32_t* id_o
     rc = commit_design(c, P, id_out);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));  // P's host vectors are read until here
+    // the transform matrices of the design's bin shape, both field modes, so
+    // a placement of it needs no allocation of its own (ep_place_noalloc)
+    const DesignDev& G = c->designs[*id_out]->dev;
+    for (int mode : {EP_FIELD_REFERENCE, EP_FIELD_IDXST}) {
+        Transform* t = nullptr;
+        rc = get_transform(c, G.Bx, mode, &t);
+        if (!rc && G.By != G.Bx) rc = get_transform(c, G.By, mode, &t);
+        if (rc) {
+            free_design_locked(c, *id_out);
+            return rc;
+        }
+    }
     return EP_OK;
 }
 

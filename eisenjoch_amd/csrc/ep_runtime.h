@@ -68,6 +68,7 @@ struct Batch {
     PlaneSum* sums = nullptr;        // split final passes (device), summed after pass 3
     int n_sums = 0, sum_blocks = 0;
     void* mem = nullptr;        // descriptors, prefixes, jobs
+    bool ext_mem = false;       // mem is a caller-owned device buffer (not freed here)
     DesignDev* d_dev = nullptr;
     const int32_t* dzero = nullptr;  // a device zero: prefix of a one-design view
     BatchDev bd{};
@@ -163,6 +164,17 @@ void release_batch_cache(Ctx* c);
 
 // Batches (ep_place.cpp)
 int batch_build(Ctx* c, const std::vector<Design*>& ds, const ep_config* cfg, Batch& b);
+int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<const ep_config*>& cfgs,
+                     Batch& b, void* ext_mem, size_t ext_cap);
+int batch_reset_some(Ctx* c, Batch& b, const std::vector<int>& idx);
+int validate_config(Ctx* c, const ep_config* cfg, int log_cap);
+// Runs a built batch until every design stops (graph-replayed chunks, or
+// eager launches without use_graph).
+int batch_place(Ctx* c, Batch& b, int sync_every, bool use_graph);
+Design* find_design(Ctx* c, uint32_t id);
+// Device address of the allocation-free entry point's buffer on the
+// context's device (ep_noalloc.hip); called once at context creation.
+int noalloc_init(Ctx* c);
 void batch_release(Batch& b);
 int batch_reset(Ctx* c, Batch& b, const ep_config* cfg);
 // fix: 1 / 0 launches the iterations with / without the fix pass; -1 decides

@@ -22,10 +22,6 @@
 
 namespace ep {
 
-int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<const ep_config*>& cfgs,
-                     Batch& b);
-int batch_reset_some(Ctx* c, Batch& b, const std::vector<int>& idx);
-int validate_config(Ctx* c, const ep_config* cfg, int log_cap);
 
 enum ReqState { kQueued = 0, kRunning = 1, kDone = 2 };
 
@@ -166,7 +162,7 @@ static void serve_loop(Ctx* c) {
                 ds.push_back(r->D);
                 cf.push_back(&r->cfg);
             }
-            if (batch_build_cfgs(c, ds, cf, S.batch)) break;
+            if (batch_build_cfgs(c, ds, cf, S.batch, nullptr, 0)) break;
             fresh.clear();
             for (size_t i = 0; i < S.active.size(); i++)
                 if (std::find(fresh_reqs.begin(), fresh_reqs.end(), S.active[i]) != fresh_reqs.end())

@@ -430,6 +430,24 @@ class Device:
             _check(rc, self._h, "ep_place_batch")
         return out
 
+    def place_noalloc(self, design_id: int, cfg: ElectroPlaceCfg, positions: bool = True,
+                      log: bool = False) -> PlaceResult:
+        """ep_place_noalloc: the same placement as place(), with no device
+        allocation (launch descriptors in the library's preallocated buffer,
+        eager launches instead of a graph)."""
+        r = L.EpResult()
+        rc = L.lib().ep_place_noalloc(self._h, design_id, C.byref(cfg.to_c()), C.byref(r))
+        if rc not in (L.EP_OK, L.EP_ERR_NONFINITE):
+            _check(rc, self._h, "ep_place_noalloc")
+        out = _result(r)
+        if positions:
+            out.x, out.y = self.positions(design_id)
+        if log:
+            out.log = self.iter_log(design_id)
+        if rc == L.EP_ERR_NONFINITE:
+            _check(rc, self._h, "ep_place_noalloc")
+        return out
+
     def positions(self, design_id: int):
         n = self.designs[design_id].n_mov
         x = np.zeros(n)

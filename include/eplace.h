@@ -149,6 +149,17 @@ int ep_place(ep_ctx* ctx, uint32_t design_id, const ep_config* cfg, ep_result* r
 /* Advances every listed design in the same launches until each stops. */
 int ep_place_batch(ep_ctx* ctx, const uint32_t* ids, uint32_t n, const ep_config* cfg,
                    ep_result* results);
+/* No counterpart in the reference.  ep_place for callers that cannot
+ * allocate device memory (serving loops kept off the allocator, harnesses
+ * that forbid it): the same placement, bit for bit, with no hipMalloc, no
+ * hipFree and no graph instantiation.  The batch's launch descriptors go
+ * into one file-scope device buffer of 1 MiB per GPU (held until the process
+ * ends), enough for designs of tens of millions of pins; a design whose
+ * descriptors do not fit is refused with EP_ERR_INVALID_ARG.  The design
+ * must have been uploaded with ep_design_upload (which prepares its
+ * transform matrices).  Calls that overlap on one GPU from different
+ * contexts race for the buffer; calls through one context are serialised. */
+int ep_place_noalloc(ep_ctx* ctx, uint32_t design_id, const ep_config* cfg, ep_result* res);
 int ep_positions(ep_ctx* ctx, uint32_t design_id, double* x, double* y);
 int ep_iter_log(ep_ctx* ctx, uint32_t design_id, ep_iter_rec* recs, uint32_t cap, uint32_t* n);
 
