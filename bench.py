@@ -222,6 +222,7 @@ def serve(args):
                   f"done {sum(tk[2] is not None for tk in tickets)}", file=sys.stderr, flush=True)
         time.sleep(0.0005)
     wall = time.perf_counter() - t0
+    st = srv.stats()
     srv.stop()
     res = [tk[2] for tk in tickets]
     lat = np.array([r.t_done_ms - r.t_submit_ms for r in res])
@@ -237,6 +238,8 @@ def serve(args):
                    "pool": len(pool), "seed": 1,
                    "stop_check": "every iteration" if args.stop == "every" else "every 5 iterations"},
         "dtype": "f32+bf16x3-mfma+f64-reductions", "data": "synthetic (seeded generator)",
+        "serving_stats": {**st, "batch_builds_per_s": round(st["batch_builds"] / wall, 2),
+                          "graph_captures_per_s": round(st["graph_captures"] / wall, 2)},
     }
     # HPWL of every served result against the plain FP64 placer on the same
     # pool design (one design at a time, after the serving window)

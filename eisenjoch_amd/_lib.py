@@ -81,6 +81,12 @@ class EpServeConfig(C.Structure):
                 ("hbm_budget_bytes", C.c_uint64)]
 
 
+class EpServeStats(C.Structure):
+    _fields_ = [("admitted", C.c_uint64), ("finished", C.c_uint64), ("batch_builds", C.c_uint64),
+                ("graph_captures", C.c_uint64), ("chunks", C.c_uint64), ("failed", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 class EpPhaseTime(C.Structure):
     _fields_ = [("name", C.c_char * 24), ("launches", C.c_int32), ("reserved", C.c_int32),
                 ("avg_ms", C.c_double)]
@@ -134,6 +140,7 @@ SIGNATURES = {
     "ep_serve_positions": (C.c_int, [_p, C.c_uint64, _p, _p]),
     "ep_serve_release": (C.c_int, [_p, C.c_uint64]),
     "ep_serve_cancel": (C.c_int, [_p, C.c_uint64]),
+    "ep_serve_stats_get": (C.c_int, [_p, C.POINTER(EpServeStats)]),
     "ep_serve_stop": (C.c_int, [_p]),
     "ep_profile_iteration": (C.c_int, [_p, C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(EpConfig),
                                        C.c_uint32, C.c_uint32, C.POINTER(EpPhaseTime), C.c_uint32,

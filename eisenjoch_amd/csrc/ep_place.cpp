@@ -493,6 +493,7 @@ int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph, int fix) {
         hipGraphDestroy(g);
         if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
         gi = iters;
+        c->graph_captures++;
     }
     HIPCHK(c, hipGraphLaunch(ge, c->stream));
     return EP_OK;

@@ -109,6 +109,7 @@ struct Ctx : ep_ctx {
     bool cache_cfg_valid = false;
     ep_config eval_cfg{};            // configuration of the ep_eval_* calls
     bool server_running = false;
+    int graph_captures = 0;          // hipGraphs instantiated by batch_run_iters
     std::map<std::pair<int, int>, int> tune;   // (Bx, By) -> transform engine (autotuned)
     std::unique_ptr<Server, ServerDeleter> server;
 };

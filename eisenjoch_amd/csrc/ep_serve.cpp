@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -51,6 +52,10 @@ struct Server {
     uint64_t next_ticket = 1;
     Batch batch;
     bool dirty = true;
+    int failed = EP_OK;        // status of the failure that stopped the serving thread
+    std::string failed_msg;
+    int64_t fail_after = -1;   // EP_SERVE_FAIL_AFTER: chunks before a forced failure (tests)
+    ep_serve_stats stats{};
 };
 
 void ServerDeleter::operator()(Server* s) const { delete s; }
@@ -99,11 +104,44 @@ static int copy_positions(Ctx* c, const Design& D, std::vector<double>& x, std::
     return EP_OK;
 }
 
+// Ends every queued and running request with status rc (after a device
+// failure the serving thread stops; the designs' memory is released).
+static void fail_all(Ctx* c, Server& S, int rc) {
+    std::string msg;
+    {
+        std::lock_guard<std::mutex> eg(c->err_mu);
+        msg = c->last_error;
+    }
+    std::vector<ServeReq*> live;
+    {
+        std::lock_guard<std::mutex> lk(S.m);
+        S.failed = rc;
+        S.failed_msg = msg;
+        for (ServeReq* r : S.active)
+            if (r) live.push_back(r);
+        for (ServeReq* r : S.queue) live.push_back(r);
+        S.active.clear();
+        S.queue.clear();
+    }
+    for (ServeReq* r : live) {
+        if (r->D) free_design_locked(c, r->D->id);
+        std::lock_guard<std::mutex> lk(S.m);
+        r->D = nullptr;
+        r->prep.reset();
+        r->res.status = rc;
+        r->res.t_done_ms = now_ms() - c->t0;
+        r->status = rc;
+        r->state = kDone;
+    }
+    batch_release(S.batch);
+}
+
 static void serve_loop(Ctx* c) {
     Server& S = *c->server;
     hipSetDevice(c->device);
     std::vector<int> fresh;  // batch indices admitted since the last rebuild
     std::vector<ServeReq*> fresh_reqs;
+    int rc = EP_OK;
     for (;;) {
         std::vector<ServeReq*> admit;
         {
@@ -138,13 +176,13 @@ static void serve_loop(Ctx* c) {
         std::lock_guard<std::recursive_mutex> g(c->mu);
         for (ServeReq* r : admit) {
             uint32_t id = 0;
-            int rc = commit_design(c, *r->prep, &id);
+            int arc = commit_design(c, *r->prep, &id);
             std::lock_guard<std::mutex> lk(S.m);
             r->res.t_admit_ms = now_ms() - c->t0;
-            if (rc) {
+            if (arc) {  // this request alone (budget, state pool)
                 r->state = kDone;
-                r->status = rc;
-                r->res.status = rc;
+                r->status = arc;
+                r->res.status = arc;
                 r->res.t_done_ms = r->res.t_admit_ms;
                 continue;
             }
@@ -153,6 +191,7 @@ static void serve_loop(Ctx* c) {
             S.active.push_back(r);
             fresh_reqs.push_back(r);
             S.dirty = true;
+            S.stats.admitted++;
         }
         if (S.active.empty()) continue;
         if (S.dirty) {
@@ -162,19 +201,28 @@ static void serve_loop(Ctx* c) {
                 ds.push_back(r->D);
                 cf.push_back(&r->cfg);
             }
-            if (batch_build_cfgs(c, ds, cf, S.batch, nullptr, 0)) break;
+            if ((rc = batch_build_cfgs(c, ds, cf, S.batch, nullptr, 0))) break;
+            S.stats.batch_builds++;
             fresh.clear();
             for (size_t i = 0; i < S.active.size(); i++)
                 if (std::find(fresh_reqs.begin(), fresh_reqs.end(), S.active[i]) != fresh_reqs.end())
                     fresh.push_back((int)i);
-            if (!fresh.empty() && batch_reset_some(c, S.batch, fresh)) break;
+            if (!fresh.empty() && (rc = batch_reset_some(c, S.batch, fresh))) break;
             fresh_reqs.clear();
             S.dirty = false;
         }
-        if (batch_run_iters(c, S.batch, S.scfg.iters_per_sync, getenv("EP_NO_GRAPH") == nullptr)) break;
-        if (read_states(c, S.batch)) break;
+        if (S.fail_after >= 0 && (int64_t)S.stats.chunks >= S.fail_after) {
+            // forced from the host before any launch of this chunk (tests of the failure path)
+            rc = fail(c, EP_ERR_DEVICE, "forced serving failure (EP_SERVE_FAIL_AFTER)");
+            break;
+        }
+        const int caps = c->graph_captures;
+        if ((rc = batch_run_iters(c, S.batch, S.scfg.iters_per_sync, getenv("EP_NO_GRAPH") == nullptr))) break;
+        S.stats.graph_captures += (uint64_t)(c->graph_captures - caps);
+        S.stats.chunks++;
+        if ((rc = read_states(c, S.batch))) break;
         bool changed = false;
-        for (int i = 0; i < (int)S.active.size(); i++) {
+        for (int i = 0; i < (int)S.active.size() && !rc; i++) {
             ServeReq* r = S.active[i];
             const DevState& st = c->state_host[r->D->state_idx];
             bool cancel;
@@ -183,9 +231,9 @@ static void serve_loop(Ctx* c) {
                 cancel = r->cancel;
             }
             if (!st.done && !cancel) continue;
-            finalize_one(c, S.batch, i);
             std::vector<double> xo, yo;
-            copy_positions(c, *r->D, xo, yo);
+            if ((rc = finalize_one(c, S.batch, i))) break;
+            if ((rc = copy_positions(c, *r->D, xo, yo))) break;
             ep_result res = r->res;
             fill_result(*r->D, c->state_host[r->D->state_idx], &res);
             if (cancel) res.status = EP_ERR_CANCELLED;
@@ -200,8 +248,10 @@ static void serve_loop(Ctx* c) {
             r->status = res.status;
             r->state = kDone;
             S.active[i] = nullptr;
+            S.stats.finished++;
             changed = true;
         }
+        if (rc) break;
         if (changed) {
             std::lock_guard<std::mutex> lk(S.m);
             S.active.erase(std::remove(S.active.begin(), S.active.end(), nullptr), S.active.end());
@@ -210,6 +260,7 @@ static void serve_loop(Ctx* c) {
         }
     }
     std::lock_guard<std::recursive_mutex> g(c->mu);
+    if (rc) fail_all(c, S, rc);
     batch_release(S.batch);
 }
 
@@ -232,6 +283,7 @@ int ep_serve_start(ep_ctx* h, const ep_serve_config* scfg) {
     if (s.iters_per_sync == 0) s.iters_per_sync = 5;
     if (s.hbm_budget_bytes) c->budget = std::min<uint64_t>(c->budget, s.hbm_budget_bytes);
     c->server->scfg = s;
+    if (const char* fa = getenv("EP_SERVE_FAIL_AFTER")) c->server->fail_after = strtoll(fa, nullptr, 10);
     c->server_running = true;
     c->server->th = std::thread(serve_loop, c);
     return EP_OK;
@@ -272,6 +324,7 @@ int ep_serve_submit(ep_ctx* h, const ep_design* d, const ep_config* cfg, uint64_
     if (r->bytes > c->budget) return fail(c, EP_ERR_ADMISSION_REFUSED, "design exceeds the HBM budget");
     Server& S = *c->server;
     std::lock_guard<std::mutex> lk(S.m);
+    if (S.failed) return fail(c, S.failed, "serving stopped after a failure: " + S.failed_msg);
     r->ticket = S.next_ticket++;
     r->res.status = EP_PENDING;
     r->res.t_submit_ms = now_ms() - c->t0;
@@ -318,6 +371,17 @@ int ep_serve_release(ep_ctx* h, uint64_t ticket) {
     if (it == S.all.end()) return fail(c, EP_ERR_NOT_FOUND, "unknown ticket");
     if (it->second->state != kDone) return fail(c, EP_ERR_INVALID_ARG, "request still pending");
     S.all.erase(it);
+    return EP_OK;
+}
+
+int ep_serve_stats_get(ep_ctx* h, ep_serve_stats* out) {
+    Ctx* c = (Ctx*)h;
+    if (!c || !out) return fail(c, EP_ERR_INVALID_ARG, "null argument");
+    if (!c->server) return fail(c, EP_ERR_INVALID_ARG, "serving not started");
+    Server& S = *c->server;
+    std::lock_guard<std::mutex> lk(S.m);
+    *out = S.stats;
+    out->failed = S.failed;
     return EP_OK;
 }
 

@@ -598,6 +598,15 @@ class Server:
     def cancel(self, ticket: int):
         _check(L.lib().ep_serve_cancel(self.dev.handle, ticket), self.dev.handle, "ep_serve_cancel")
 
+    def stats(self) -> dict:
+        """Serving counters (ep_serve_stats_get): admissions, finishes, batch
+        rebuilds, graph captures, chunks, and the status of a failure that
+        stopped the serving thread (0 if none)."""
+        st = L.EpServeStats()
+        _check(L.lib().ep_serve_stats_get(self.dev.handle, C.byref(st)), self.dev.handle,
+               "ep_serve_stats_get")
+        return {f[0]: getattr(st, f[0]) for f in L.EpServeStats._fields_ if f[0] != "reserved"}
+
     def stop(self):
         L.lib().ep_serve_stop(self.dev.handle)
 

@@ -198,6 +198,20 @@ int ep_serve_poll(ep_ctx* ctx, uint64_t ticket, ep_result* res);
 int ep_serve_positions(ep_ctx* ctx, uint64_t ticket, double* x, double* y);
 int ep_serve_release(ep_ctx* ctx, uint64_t ticket);
 int ep_serve_cancel(ep_ctx* ctx, uint64_t ticket);
+/* Counters of the serving thread (no counterpart in the reference).  After a
+ * failure inside the serving thread (a device error), every queued and
+ * running request ends with that status, `failed` holds it and later
+ * ep_serve_submit calls return it. */
+typedef struct ep_serve_stats {
+    uint64_t admitted;        /* requests admitted into the running batch */
+    uint64_t finished;        /* requests that left it (done or cancelled) */
+    uint64_t batch_builds;    /* batch rebuilds after membership changes */
+    uint64_t graph_captures;  /* iteration graphs captured and instantiated */
+    uint64_t chunks;          /* chunks of iterations launched */
+    int32_t failed;           /* 0, or the status that stopped the serving thread */
+    int32_t reserved;
+} ep_serve_stats;
+int ep_serve_stats_get(ep_ctx* ctx, ep_serve_stats* out);
 int ep_serve_stop(ep_ctx* ctx);
 
 /* ---- measurement ---- */

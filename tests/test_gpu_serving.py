@@ -116,3 +116,41 @@ def test_serving_rejects_invalid_designs_at_submit():
             assert wait(srv, [t])[t].status == 0
         finally:
             srv.stop()
+
+
+def test_serving_failure_ends_every_ticket(monkeypatch):
+    # A failure inside the serving thread (forced from the host before the
+    # fourth chunk launches, so no kernel is involved) must end every queued
+    # and running request with the error, refuse later submissions and leave
+    # nothing pending.  A context needs a device, hence the gpu mark.
+    ds = [ep.sized_design(n, 500 + k) for k, n in enumerate([3000, 4000, 2500, 5000, 3500])]
+    monkeypatch.setenv("EP_SERVE_FAIL_AFTER", "3")
+    with ep.Device(0) as dev:
+        log_case("serving: forced failure after 3 chunks")
+        srv = ep.Server(dev, max_slots=2, iters_per_sync=5)
+        try:
+            tickets = [srv.submit(d, CFG) for d in ds]
+            res = wait(srv, tickets, timeout=60.0)  # nothing hangs
+            # 15 iterations ran before the failure: no design can have stopped (min 20)
+            assert all(res[t].status == ep._lib.EP_ERR_DEVICE for t in tickets)
+            with pytest.raises(ep.DeviceError, match="forced serving failure"):
+                srv.submit(ds[0], CFG)
+            st = srv.stats()
+            assert st["failed"] == ep._lib.EP_ERR_DEVICE and st["chunks"] == 3
+            assert st["finished"] == 0 and st["admitted"] == 2
+            with pytest.raises(ep.PlacerError):
+                srv.positions(tickets[0])
+            for t in tickets:
+                srv.release(t)
+        finally:
+            srv.stop()
+    # a new context serves normally again
+    monkeypatch.delenv("EP_SERVE_FAIL_AFTER")
+    with ep.Device(0) as dev:
+        srv = ep.Server(dev, max_slots=2)
+        try:
+            t = srv.submit(ds[2], CFG)
+            assert wait(srv, [t])[t].status == 0
+            assert srv.stats()["failed"] == 0
+        finally:
+            srv.stop()

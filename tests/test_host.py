@@ -40,7 +40,7 @@ def test_struct_layouts_match_header(tmp_path):
     structs = {"ep_design": L.EpDesign, "ep_config": L.EpConfig, "ep_iter_rec": L.EpIterRec,
                "ep_result": L.EpResult, "ep_serve_config": L.EpServeConfig,
                "ep_gen_params": L.EpGenParams, "ep_gen_design": L.EpGenDesign,
-               "ep_phase_time": L.EpPhaseTime}
+               "ep_phase_time": L.EpPhaseTime, "ep_serve_stats": L.EpServeStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "eplace.h"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
