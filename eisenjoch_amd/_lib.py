@@ -76,6 +76,18 @@ class EpResult(C.Structure):
     ]
 
 
+class EpLegalProblem(C.Structure):
+    _fields_ = [("n_bels", C.c_int32), ("n_children", C.c_int32), ("bel_x", _p), ("bel_y", _p),
+                ("bel_bucket", _p), ("bel_taken", _p), ("cell_bucket", _p), ("child_root", _p),
+                ("child_dx", _p), ("child_dy", _p), ("child_bucket", _p), ("child_fixed", _p),
+                ("net_has_driver", _p)]
+
+
+class EpLegalResult(C.Structure):
+    _fields_ = [("best_iter", C.c_int32), ("legalizations", C.c_int32), ("best_hpwl", C.c_double),
+                ("final_displacement", C.c_double)]
+
+
 class EpServeConfig(C.Structure):
     _fields_ = [("max_slots", C.c_uint32), ("iters_per_sync", C.c_uint32),
                 ("hbm_budget_bytes", C.c_uint64)]
@@ -150,6 +162,10 @@ SIGNATURES = {
     "ep_tune_table": (C.c_int, [_p, C.POINTER(C.c_int32), C.c_uint32, C.POINTER(C.c_uint32)]),
     "ep_tune_set": (C.c_int, [_p, C.c_int32, C.c_int32, C.c_int32]),
     "ep_legalize": (C.c_int, [C.c_int32, _p, _p, _p, C.c_int32, _p, _p, _p, _p, _p, _p]),
+    "ep_legalize_problem": (C.c_int, [C.POINTER(EpLegalProblem), C.c_int32, _p, _p, _p, _p, _p]),
+    "ep_legal_hpwl": (C.c_int, [C.POINTER(EpDesign), C.POINTER(EpLegalProblem), _p, _p, _p]),
+    "ep_place_legalized": (C.c_int, [_p, C.c_uint32, C.POINTER(EpConfig), C.POINTER(EpLegalProblem),
+                                     C.POINTER(EpResult), C.POINTER(EpLegalResult), _p, _p]),
     "ep_generate": (C.c_int, [C.POINTER(EpGenParams), C.POINTER(C.POINTER(EpGenDesign))]),
     "ep_gen_free": (None, [C.POINTER(EpGenDesign)]),
 }

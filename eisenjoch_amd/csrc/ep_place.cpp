@@ -594,7 +594,7 @@ Design* find_design(Ctx* c, uint32_t id) {
 
 // ------------------------------------------------------------ caching ----
 
-static Batch* cached_batch(Ctx* c, const std::vector<Design*>& ds, const ep_config* cfg, int* rc) {
+Batch* cached_batch(Ctx* c, const std::vector<Design*>& ds, const ep_config* cfg, int* rc) {
     *rc = EP_OK;
     if (c->cache && c->cache->ds == ds && c->cache_cfg_valid &&
         memcmp(&c->cache_cfg, cfg, sizeof(ep_config)) == 0)

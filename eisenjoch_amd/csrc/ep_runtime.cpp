@@ -508,6 +508,10 @@ int ep_design_upload(ep_ctx* h, const ep_design* d, int32_t prec, uint32_t* id_o
     Prepared P;
     int rc = prepare_design(c, d, prec, kDefaultLogCap, P);  // host work outside the lock
     if (rc) return rc;
+    if (d->n_nets > 0) {
+        P.D->net_off_h.assign(d->net_off, d->net_off + d->n_nets + 1);
+        P.D->pin_cell_h.assign(d->pin_cell, d->pin_cell + d->n_pins);
+    }
     std::lock_guard<std::recursive_mutex> g(c->mu);
     rc = commit_design(c, P, id_out);
     if (rc) return rc;
@@ -556,6 +560,8 @@ int prepare_design(Ctx* c, const ep_design* d, int prec, int log_cap, Prepared& 
     const size_t N = (size_t)d->n_mov, F = (size_t)d->n_fixed;
     to_prec(P.x0[0], d->x0, N, D.prec);
     to_prec(P.x0[1], d->y0, N, D.prec);
+    D.fixed0[0].assign(d->fixed_x, d->fixed_x + F);
+    D.fixed0[1].assign(d->fixed_y, d->fixed_y + F);
     to_prec(P.fixed[0], d->fixed_x, F, D.prec);
     to_prec(P.fixed[1], d->fixed_y, F, D.prec);
     {

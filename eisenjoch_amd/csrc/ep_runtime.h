@@ -47,6 +47,10 @@ struct Design {
     size_t cs_count = 0;            // movable pins of nets of <= 64 pins (cell -> slot CSR entries)
     size_t chf_count = 0;           // movable pins of high-fanout nets
     bool weighted = false;          // per-net weights given (slot_w allocated)
+    std::vector<double> fixed0[2];  // the fixed cells' uploaded locations (restored after a
+                                    // best-snapshot placement moved cluster children)
+    std::vector<uint32_t> net_off_h, pin_cell_h;  // host copy of the net CSR (ep_design_upload
+                                                  // only: the legalized HPWL reads it)
     DesignDev dev{};
 };
 
@@ -172,6 +176,10 @@ int validate_config(Ctx* c, const ep_config* cfg, int log_cap);
 // Runs a built batch until every design stops (graph-replayed chunks, or
 // eager launches without use_graph).
 int batch_place(Ctx* c, Batch& b, int sync_every, bool use_graph);
+// The batch cached for one design list and configuration (rebuilt on a change).
+Batch* cached_batch(Ctx* c, const std::vector<Design*>& ds, const ep_config* cfg, int* rc);
+// Writes the fixed cells' locations of D (both copies the kernels read).
+int write_fixed(Ctx* c, Design& D, const double* fx, const double* fy);
 Design* find_design(Ctx* c, uint32_t id);
 // Device address of the allocation-free entry point's buffer on the
 // context's device (ep_noalloc.hip); called once at context creation.

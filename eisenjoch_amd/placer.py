@@ -198,6 +198,86 @@ class FlatDesign:
         return int(L.lib().ep_design_bytes(C.byref(self.to_c()), _prec(precision)))
 
 
+@dataclass
+class LegalProblem:
+    """Legalization view of a design (include/eplace.h ep_legal_problem):
+    BELs with their buckets (bucket order = BEL index order,
+    context/buckets.rs:25-43), the bucket of each movable cell, BELs bound by
+    cells the placer never unbinds, and the cluster children placed with
+    their roots (common.rs:319-374).  A child with pins is the fixed cell
+    `child_fixed` of the design (-1 without pins)."""
+
+    bel_x: np.ndarray
+    bel_y: np.ndarray
+    bel_bucket: np.ndarray
+    cell_bucket: np.ndarray
+    bel_taken: Optional[np.ndarray] = None
+    child_root: Optional[np.ndarray] = None
+    child_dx: Optional[np.ndarray] = None
+    child_dy: Optional[np.ndarray] = None
+    child_bucket: Optional[np.ndarray] = None
+    child_fixed: Optional[np.ndarray] = None
+    net_has_driver: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        i32 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+        u8 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.uint8)
+        self.bel_x, self.bel_y = i32(self.bel_x), i32(self.bel_y)
+        self.bel_bucket, self.cell_bucket = i32(self.bel_bucket), i32(self.cell_bucket)
+        self.bel_taken, self.net_has_driver = u8(self.bel_taken), u8(self.net_has_driver)
+        n = 0 if self.child_root is None else len(self.child_root)
+        z = np.zeros(n, np.int32)
+        self.child_root = i32(self.child_root) if n else z
+        self.child_dx = i32(self.child_dx) if n else z
+        self.child_dy = i32(self.child_dy) if n else z
+        self.child_bucket = i32(self.child_bucket) if n else z
+        self.child_fixed = i32(self.child_fixed) if n else z
+
+    @property
+    def n_children(self) -> int:
+        return int(self.child_root.shape[0])
+
+    def to_c(self) -> L.EpLegalProblem:
+        c = L.EpLegalProblem()
+        c.n_bels, c.n_children = int(self.bel_x.shape[0]), self.n_children
+        c.bel_x, c.bel_y, c.bel_bucket = L.ptr(self.bel_x), L.ptr(self.bel_y), L.ptr(self.bel_bucket)
+        c.bel_taken, c.cell_bucket = L.ptr(self.bel_taken), L.ptr(self.cell_bucket)
+        c.child_root, c.child_dx, c.child_dy = L.ptr(self.child_root), L.ptr(self.child_dx), L.ptr(self.child_dy)
+        c.child_bucket, c.child_fixed = L.ptr(self.child_bucket), L.ptr(self.child_fixed)
+        c.net_has_driver = L.ptr(self.net_has_driver)
+        return c
+
+    @staticmethod
+    def of(d: "FlatDesign") -> "LegalProblem":
+        """The generator's BEL grid of a design (no clusters)."""
+        if d.bel_x is None:
+            raise PlacementFailed("design has no BEL grid to legalize onto")
+        return LegalProblem(d.bel_x, d.bel_y, d.bel_type, d.cell_type)
+
+
+def legalize_problem(lp: LegalProblem, x, y):
+    """legalize_electro with cluster children (ep_legalize_problem).
+    Returns (cell_bel, child_bel, displacement)."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    cb = np.full(x.shape[0], -1, np.int32)
+    kb = np.full(lp.n_children, -1, np.int32)
+    disp = C.c_double()
+    _check(L.lib().ep_legalize_problem(C.byref(lp.to_c()), x.shape[0], L.ptr(x), L.ptr(y), L.ptr(cb),
+                                       L.ptr(kb), C.byref(disp)), None, "ep_legalize_problem")
+    return cb, kb, disp.value
+
+
+def legal_hpwl(d: "FlatDesign", lp: LegalProblem, cell_bel, child_bel=None) -> float:
+    """total_hpwl (metrics/wirelength.rs:10-51) of a legalized binding (ep_legal_hpwl)."""
+    cb = np.ascontiguousarray(cell_bel, np.int32)
+    kb = np.ascontiguousarray(child_bel if child_bel is not None else np.zeros(0), np.int32)
+    h = C.c_double()
+    _check(L.lib().ep_legal_hpwl(C.byref(d.to_c()), C.byref(lp.to_c()), L.ptr(cb), L.ptr(kb), C.byref(h)),
+           None, "ep_legal_hpwl")
+    return h.value
+
+
 def _prec(p: str) -> int:
     return {"fp64": L.EP_PREC_FP64, "fast": L.EP_PREC_FAST}[p]
 
@@ -448,6 +528,28 @@ class Device:
             _check(rc, self._h, "ep_place_noalloc")
         return out
 
+    def place_legalized(self, design_id: int, cfg: ElectroPlaceCfg, lp: LegalProblem,
+                        log: bool = False):
+        """place_electro end to end (ep_place_legalized): the continuous loop on
+        the device, a legalized and scored snapshot at every check iteration,
+        and the final binding legalize(best).  Returns (result, cell_bel,
+        child_bel, info) with info = {best_iter, legalizations, best_hpwl,
+        final_displacement}."""
+        n = self.designs[design_id].n_mov
+        cb = np.full(n, -1, np.int32)
+        kb = np.full(lp.n_children, -1, np.int32)
+        r = L.EpResult()
+        lr = L.EpLegalResult()
+        _check(L.lib().ep_place_legalized(self._h, design_id, C.byref(cfg.to_c()), C.byref(lp.to_c()),
+                                          C.byref(r), C.byref(lr), L.ptr(cb), L.ptr(kb)),
+               self._h, "ep_place_legalized")
+        out = _result(r)
+        out.x, out.y = self.positions(design_id)
+        if log:
+            out.log = self.iter_log(design_id)
+        info = {f[0]: getattr(lr, f[0]) for f in L.EpLegalResult._fields_}
+        return out, cb, kb, info
+
     def positions(self, design_id: int):
         n = self.designs[design_id].n_mov
         x = np.zeros(n)
@@ -665,24 +767,44 @@ class PlacerElectro:
         keep = np.zeros(ctx.design.n_mov, bool)
         keep[np.asarray(cells, np.int64)] = True
         sub, back = ctx.design_subset(keep)
-        sub_ctx = Context(sub)
+        legal = None
+        if ctx.legal is not None:
+            # the others keep their BELs (locked for the call, common.rs:502-530)
+            taken = (np.zeros(len(ctx.legal.bel_x), np.uint8) if ctx.legal.bel_taken is None
+                     else ctx.legal.bel_taken.copy())
+            if ctx.cell_bel is not None:
+                taken[ctx.cell_bel[~keep]] = 1
+            legal = LegalProblem(ctx.legal.bel_x, ctx.legal.bel_y, ctx.legal.bel_bucket,
+                                 ctx.legal.cell_bucket[keep], bel_taken=taken)
+        sub_ctx = Context(sub, legal=legal)
         place_electro(sub_ctx, cfg, device=self.device, precision=self.precision)
         ctx.x[back] = sub_ctx.x
         ctx.y[back] = sub_ctx.y
-        ctx.legalize()
+        if legal is not None:
+            if ctx.cell_bel is None:
+                ctx.cell_bel = np.full(ctx.design.n_mov, -1, np.int32)
+            ctx.cell_bel[back] = sub_ctx.cell_bel
 
 
 class Context:
     """Flattened stand-in for the reference Context at the placer boundary:
-    continuous positions, BEL bindings of the movable cells, and `place()`
-    with the reference's string selection (crates/npnr-python/src/lib.rs:245-306)."""
+    continuous positions, the BEL bindings of the movable cells and cluster
+    children, and `place()` with the reference's string selection
+    (crates/npnr-python/src/lib.rs:245-306).  `legal` is the legalization
+    view (BELs, buckets, cluster children); by default the design's own BEL
+    grid."""
 
-    def __init__(self, design: FlatDesign, device: Optional[Device] = None):
+    def __init__(self, design: FlatDesign, device: Optional[Device] = None,
+                 legal: Optional[LegalProblem] = None):
         self.design = design
         self.device = device
+        self.legal = legal if legal is not None else (
+            LegalProblem.of(design) if design.bel_x is not None else None)
         self.x = design.x0.copy()
         self.y = design.y0.copy()
         self.cell_bel = design.init_bel.copy() if design.init_bel is not None else None
+        self.child_bel = None
+        self.best: Optional[dict] = None  # best snapshot of the last placement
         self.last_result: Optional[PlaceResult] = None
 
     def place(self, *, placer: str = "heap", seed: int = 1, max_iters: Optional[int] = None,
@@ -698,17 +820,17 @@ class Context:
             raise RuntimeError(f"ElectroPlace error: {e}") from e
 
     def legalize(self) -> float:
-        d = self.design
-        if d.bel_x is None:
+        if self.legal is None:
             raise PlacementFailed("design has no BEL grid to legalize onto")
-        self.cell_bel, disp = legalize(self.x, self.y, d.cell_type, d.bel_x, d.bel_y, d.bel_type)
+        self.cell_bel, self.child_bel, disp = legalize_problem(self.legal, self.x, self.y)
         return disp
 
     def total_hpwl(self) -> float:
-        d = self.design
-        if self.cell_bel is None:
+        """metrics/wirelength.rs:46-51 on the current binding."""
+        if self.cell_bel is None or self.legal is None:
             return 0.0
-        return total_hpwl_int(d, d.bel_x[self.cell_bel], d.bel_y[self.cell_bel])
+        kb = self.child_bel if self.child_bel is not None else np.zeros(self.legal.n_children, np.int32)
+        return legal_hpwl(self.design, self.legal, self.cell_bel, kb)
 
     def design_subset(self, keep: np.ndarray):
         """Design whose movable cells are `keep`; the others become fixed at
@@ -734,12 +856,12 @@ class Context:
 
 def place_electro(ctx: Context, cfg: ElectroPlaceCfg, device: Optional[Device] = None,
                   precision: str = "fast") -> PlaceResult:
-    """mod.rs:65-204 with the continuous loop on the GPU: every
-    `legalize_interval` iterations the reference legalizes, records the
-    legalized HPWL and keeps the best positions; the final placement
-    legalizes the best snapshot.  Here the GPU runs the continuous loop (its
-    trajectory does not depend on legalization, which never writes back to
-    the solver) and the host legalizes the final positions."""
+    """mod.rs:65-204.  With a legalization view (the usual case) the whole
+    of it runs through ep_place_legalized: the continuous loop on the GPU,
+    at every check iteration (`legalize_interval`, the last one, and a stop)
+    the positions legalized on the host and scored by total_hpwl, the best
+    kept (common.rs:485-495), and the final binding legalize(best)
+    (mod.rs:199).  Without one, only the continuous loop runs."""
     d = ctx.design
     if d.n_mov == 0:
         return PlaceResult(0, 0, False, 0.0, 0.0, 0.0)  # mod.rs:80-82
@@ -748,7 +870,11 @@ def place_electro(ctx: Context, cfg: ElectroPlaceCfg, device: Optional[Device] =
     try:
         i = dev.upload(d, precision)
         try:
-            r = dev.place(i, cfg, positions=True, log=True)
+            if ctx.legal is not None:
+                r, cb, kb, info = dev.place_legalized(i, cfg, ctx.legal, log=True)
+                ctx.cell_bel, ctx.child_bel, ctx.best = cb, kb, info
+            else:
+                r = dev.place(i, cfg, positions=True, log=True)
         finally:
             dev.free(i)
     finally:
@@ -756,8 +882,6 @@ def place_electro(ctx: Context, cfg: ElectroPlaceCfg, device: Optional[Device] =
             dev.close()
     ctx.x, ctx.y = r.x.copy(), r.y.copy()
     ctx.last_result = r
-    if d.bel_x is not None:
-        ctx.legalize()
     if r.status == L.EP_ERR_NONFINITE:
         raise NonFiniteError("non-finite placement state")
     return r

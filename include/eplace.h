@@ -261,6 +261,64 @@ int ep_legalize(int32_t n_cells, const double* x, const double* y, const int32_t
                 const int32_t* bel_bucket, uint8_t* bel_taken, int32_t* cell_bel,
                 double* displacement);
 
+/* The legalization view of a design: what `legalize_electro` (mod.rs:206-256)
+ * binds and what `total_hpwl` (metrics/wirelength.rs:10-51) measures.
+ * Buckets are the cell types; a bucket's BELs in ascending index order are
+ * its bucket order (context/buckets.rs:25-43).  Cluster children
+ * (common.rs:319-374) are the non-root cells of movable clusters: the solver
+ * does not move them, legalization places each right after its root, at
+ * root location + (dx, dy) when a free BEL of its bucket is there, else on
+ * the first free BEL of its bucket.  A child with pins is one of the
+ * design's fixed cells (child_fixed; -1 without pins), whose location the
+ * WA gradient reads, so it follows the child after every legalization. */
+typedef struct ep_legal_problem {
+    int32_t n_bels;
+    int32_t n_children;
+    const int32_t* bel_x;         /* [n_bels] */
+    const int32_t* bel_y;         /* [n_bels] */
+    const int32_t* bel_bucket;    /* [n_bels] */
+    const uint8_t* bel_taken;     /* [n_bels] or NULL: BELs bound by cells the placer never
+                                     unbinds (locked cells) */
+    const int32_t* cell_bucket;   /* [n_mov] bucket of each movable cell */
+    const int32_t* child_root;    /* [n_children] solver index of the root; roots ascending,
+                                     a root's children in their constraint order */
+    const int32_t* child_dx;      /* [n_children] constr_x */
+    const int32_t* child_dy;      /* [n_children] constr_y */
+    const int32_t* child_bucket;  /* [n_children] */
+    const int32_t* child_fixed;   /* [n_children] fixed-cell index or -1 */
+    const uint8_t* net_has_driver; /* [n_nets] or NULL (every net has one): driverless nets
+                                      count 0 in the legalized HPWL (wirelength.rs:16-18) */
+} ep_legal_problem;
+
+/* legalize_electro on an ep_legal_problem (mod.rs:206-256 with
+ * place_cluster_children, common.rs:319-374); child_bel may be NULL when
+ * n_children is 0.  EP_ERR_NO_BELS / EP_ERR_PLACEMENT_FAILED as the
+ * reference's PlacerError. */
+int ep_legalize_problem(const ep_legal_problem* lp, int32_t n_cells, const double* x, const double* y,
+                        int32_t* cell_bel, int32_t* child_bel, double* displacement);
+/* total_hpwl (metrics/wirelength.rs:10-51) of a legalized binding of d. */
+int ep_legal_hpwl(const ep_design* d, const ep_legal_problem* lp, const int32_t* cell_bel,
+                  const int32_t* child_bel, double* hpwl);
+
+typedef struct ep_legal_result {
+    int32_t best_iter;         /* iteration of the best snapshot; -1: the initial positions */
+    int32_t legalizations;     /* snapshots legalized and scored */
+    double best_hpwl;          /* its legalized HPWL (+inf when none was scored) */
+    double final_displacement; /* of legalize(best) */
+} ep_legal_result;
+
+/* place_electro (mod.rs:65-204) end to end on a resident design: the
+ * continuous loop on the device, and at every check iteration (iter %
+ * legalize_interval == 0, the last iteration, and the iteration a design
+ * stops at) the positions legalized on the host, scored by total_hpwl and
+ * kept when strictly better (NesterovLoopState::record_metric,
+ * common.rs:485-495); cluster children with pins move to their new BELs
+ * before the next iteration.  The final binding is legalize(best)
+ * (mod.rs:199) into cell_bel [n_mov] and child_bel [n_children]. */
+int ep_place_legalized(ep_ctx* ctx, uint32_t design_id, const ep_config* cfg,
+                       const ep_legal_problem* lp, ep_result* res, ep_legal_result* lres,
+                       int32_t* cell_bel, int32_t* child_bel);
+
 /* Synthetic generic-FPGA design generator (clustered-locality netlists). */
 typedef struct ep_gen_params {
     uint64_t seed;

@@ -750,8 +750,14 @@ static double grad_norm(const double* gx, const double* gy, int n) {
     return sqrt(s);
 }
 
-int orc_place(const orc_design* d, const orc_cfg* cfg, double* x_out, double* y_out,
-              orc_iter_rec* log, int log_cap, int* n_log, orc_result* res) {
+/* Called on the legalization iterations of mod.rs:176 (and on the one a
+ * per-iteration stop rule ends at) with the solver positions after the step;
+ * a nonzero return aborts the loop with that status. */
+typedef int (*orc_check_fn)(void* user, int iter, const double* x, const double* y);
+
+static int place_core(const orc_design* d, const orc_cfg* cfg, double* x_out, double* y_out,
+                      orc_iter_rec* log, int log_cap, int* n_log, orc_result* res,
+                      orc_check_fn on_check, void* user) {
     int n = d->n_mov;
     if (n_log) *n_log = 0;
     memset(res, 0, sizeof(*res));
@@ -843,7 +849,16 @@ int orc_place(const orc_design* d, const orc_cfg* cfg, double* x_out, double* y_
             r->wl_norm = wln;
             r->den_norm = dnn;
         }
-        if (check && it >= cfg->min_iters && ov < cfg->overflow_target) { /* mod.rs:192-195 */
+        int stop = check && it >= cfg->min_iters && ov < cfg->overflow_target;
+        int leg = it % cfg->legalize_interval == 0 || it == cfg->max_iters - 1;
+        if (on_check && (leg || stop)) { /* mod.rs:176-183: legalize, HPWL, record_metric */
+            int rc = on_check(user, it, nx.x, ny.x);
+            if (rc) {
+                free(mem);
+                return rc;
+            }
+        }
+        if (stop) { /* mod.rs:192-195 */
             converged = 1;
             it++;
             break;
@@ -859,4 +874,173 @@ int orc_place(const orc_design* d, const orc_cfg* cfg, double* x_out, double* y_
     if (n_log) *n_log = nlog;
     free(mem);
     return 0;
+}
+
+int orc_place(const orc_design* d, const orc_cfg* cfg, double* x_out, double* y_out,
+              orc_iter_rec* log, int log_cap, int* n_log, orc_result* res) {
+    return place_core(d, cfg, x_out, y_out, log, log_cap, n_log, res, NULL, NULL);
+}
+
+/* ------------------------------------------------------------------ */
+/* legalization with cluster children and the best snapshot            */
+/* ------------------------------------------------------------------ */
+
+/* mod.rs:206-256 with place_cluster_children (common.rs:319-374): every
+ * movable cell in solver order takes the first minimum of the squared
+ * distance over the free BELs of its bucket, scanned in bucket (BEL index)
+ * order; right after it, each of its cluster children binds the first free
+ * BEL of the child's bucket at root location + (dx, dy), else the first free
+ * BEL of that bucket.  taken holds the BELs bound on entry and is updated.
+ * Returns 0, -2 (NoBelsAvailable) or -3 (PlacementFailed). */
+int orc_legalize_full(const orc_design* d, const orc_legal* lp, const double* x, const double* y,
+                      uint8_t* taken, int32_t* cell_bel, int32_t* child_bel, double* displacement) {
+    double disp = 0.0;
+    int ch = 0;
+    for (int i = 0; i < d->n_mov; i++) {
+        int best = -1;
+        double bc = INFINITY;
+        for (int b = 0; b < lp->n_bels; b++) {
+            if (lp->bel_bucket[b] != lp->cell_bucket[i] || taken[b]) continue;
+            double dx = (double)lp->bel_x[b] - x[i];
+            double dy = (double)lp->bel_y[b] - y[i];
+            double c = dx * dx + dy * dy;
+            if (c < bc) { bc = c; best = b; }
+        }
+        if (best < 0) return -2;
+        taken[best] = 1;
+        cell_bel[i] = best;
+        disp += bc;
+        for (; ch < lp->n_children && lp->child_root[ch] == i; ch++) {
+            int tx = lp->bel_x[best] + lp->child_dx[ch], ty = lp->bel_y[best] + lp->child_dy[ch];
+            int cb = -1;
+            for (int b = 0; b < lp->n_bels && cb < 0; b++)
+                if (lp->bel_bucket[b] == lp->child_bucket[ch] && !taken[b] && lp->bel_x[b] == tx &&
+                    lp->bel_y[b] == ty)
+                    cb = b;
+            for (int b = 0; b < lp->n_bels && cb < 0; b++)
+                if (lp->bel_bucket[b] == lp->child_bucket[ch] && !taken[b]) cb = b;
+            if (cb < 0) return -3;
+            taken[cb] = 1;
+            child_bel[ch] = cb;
+        }
+    }
+    if (displacement) *displacement = disp;
+    return 0;
+}
+
+/* total_hpwl (wirelength.rs:10-51) of a binding: per net with a driver and
+ * a user, (max - min) x + y over its placed pins; movable cells at their
+ * BELs, cluster children at theirs, other fixed cells at their locations. */
+double orc_legal_hpwl(const orc_design* d, const orc_legal* lp, const int32_t* cell_bel,
+                      const int32_t* child_bel) {
+    int32_t* fx = (int32_t*)malloc(sizeof(int32_t) * (size_t)(d->n_fixed + 1));
+    int32_t* fy = (int32_t*)malloc(sizeof(int32_t) * (size_t)(d->n_fixed + 1));
+    for (int j = 0; j < d->n_fixed; j++) {
+        fx[j] = (int32_t)d->fixed_x[j];
+        fy[j] = (int32_t)d->fixed_y[j];
+    }
+    for (int k = 0; k < lp->n_children; k++)
+        if (lp->child_fixed[k] >= 0) {
+            fx[lp->child_fixed[k]] = lp->bel_x[child_bel[k]];
+            fy[lp->child_fixed[k]] = lp->bel_y[child_bel[k]];
+        }
+    double total = 0.0;
+    for (int e = 0; e < d->n_nets; e++) {
+        uint32_t b = d->net_off[e], n = d->net_off[e + 1] - b;
+        if (n < 2) continue;
+        if (lp->net_has_driver && !lp->net_has_driver[e]) continue;
+        int32_t xmn = 0x7fffffff, xmx = -0x7fffffff - 1, ymn = 0x7fffffff, ymx = -0x7fffffff - 1;
+        for (uint32_t k = 0; k < n; k++) {
+            uint32_t c = d->pin_cell[b + k];
+            int32_t px, py;
+            if (c < (uint32_t)d->n_mov) { px = lp->bel_x[cell_bel[c]]; py = lp->bel_y[cell_bel[c]]; }
+            else { px = fx[c - d->n_mov]; py = fy[c - d->n_mov]; }
+            if (px < xmn) xmn = px;
+            if (px > xmx) xmx = px;
+            if (py < ymn) ymn = py;
+            if (py > ymx) ymx = py;
+        }
+        total += (double)((xmx - xmn) + (ymx - ymn));
+    }
+    free(fx);
+    free(fy);
+    return total;
+}
+
+typedef struct {
+    const orc_design* d;       /* fixed_x / fixed_y point at fx / fy */
+    const orc_legal* lp;
+    double *fx, *fy;           /* the fixed locations the loop reads (children move) */
+    uint8_t* taken;
+    int32_t *cb, *kb;
+    double *bx, *by;           /* best positions */
+    double best;
+    int best_iter, count;
+} leg_state;
+
+static int leg_check(void* user, int it, const double* x, const double* y) {
+    leg_state* S = (leg_state*)user;
+    const orc_design* d = S->d;
+    const orc_legal* lp = S->lp;
+    /* mod.rs:177-179: positions of the solvers, clamped (already in range) */
+    for (int b = 0; b < lp->n_bels; b++) S->taken[b] = lp->bel_taken ? lp->bel_taken[b] : 0;
+    int rc = orc_legalize_full(d, lp, x, y, S->taken, S->cb, S->kb, NULL);
+    if (rc) return rc;
+    S->count++;
+    /* the children now sit on their new BELs, for the HPWL and the next iterations */
+    for (int k = 0; k < lp->n_children; k++)
+        if (lp->child_fixed[k] >= 0) {
+            S->fx[lp->child_fixed[k]] = lp->bel_x[S->kb[k]];
+            S->fy[lp->child_fixed[k]] = lp->bel_y[S->kb[k]];
+        }
+    double h = orc_legal_hpwl(d, lp, S->cb, S->kb); /* wirelength.rs:46-51 */
+    if (h < S->best) { /* common.rs:485-495 */
+        S->best = h;
+        S->best_iter = it;
+        memcpy(S->bx, x, sizeof(double) * d->n_mov);
+        memcpy(S->by, y, sizeof(double) * d->n_mov);
+    }
+    return 0;
+}
+
+/* place_electro (mod.rs:65-204) from the supplied initial positions: the
+ * continuous loop, at each legalization iteration the legalization, its
+ * HPWL and the best snapshot, then the final binding legalize(best)
+ * (mod.rs:199).  Returns 0 or the legalization's status. */
+int orc_place_legalized(const orc_design* d0, const orc_cfg* cfg, const orc_legal* lp,
+                        int32_t* cell_bel, int32_t* child_bel, double* best_hpwl, int* best_iter,
+                        int* legalizations, orc_result* res) {
+    int n = d0->n_mov;
+    orc_design d = *d0;
+    leg_state S;
+    memset(&S, 0, sizeof(S));
+    S.fx = (double*)malloc(sizeof(double) * (size_t)(d0->n_fixed + 1));
+    S.fy = (double*)malloc(sizeof(double) * (size_t)(d0->n_fixed + 1));
+    for (int j = 0; j < d0->n_fixed; j++) {
+        S.fx[j] = d0->fixed_x[j];
+        S.fy[j] = d0->fixed_y[j];
+    }
+    d.fixed_x = S.fx;
+    d.fixed_y = S.fy;
+    S.d = &d;
+    S.lp = lp;
+    S.taken = (uint8_t*)malloc((size_t)lp->n_bels + 1);
+    S.cb = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    S.kb = (int32_t*)malloc(sizeof(int32_t) * (size_t)(lp->n_children + 1));
+    S.bx = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    S.by = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    memcpy(S.bx, d0->x0, sizeof(double) * n); /* NesterovLoopState::new (common.rs:459-468) */
+    memcpy(S.by, d0->y0, sizeof(double) * n);
+    S.best = INFINITY;
+    S.best_iter = -1;
+    int rc = place_core(&d, cfg, NULL, NULL, NULL, 0, NULL, res, leg_check, &S);
+    if (!rc) {
+        for (int b = 0; b < lp->n_bels; b++) S.taken[b] = lp->bel_taken ? lp->bel_taken[b] : 0;
+        rc = orc_legalize_full(&d, lp, S.bx, S.by, S.taken, cell_bel, child_bel, NULL);
+    }
+    *best_hpwl = S.best;
+    *best_iter = S.best_iter;
+    *legalizations = S.count;
+    free(S.fx); free(S.fy); free(S.taken); free(S.cb); free(S.kb); free(S.bx); free(S.by);
+    return rc;
 }

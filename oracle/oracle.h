@@ -132,6 +132,31 @@ int orc_legalize(int n_cells, const double* x, const double* y, const int32_t* c
                  double* displacement);
 double orc_total_hpwl_int(const orc_design* d, const int32_t* cell_ix, const int32_t* cell_iy);
 
+/* ---- place_electro end to end: legalization with cluster children, the
+ *      legalized HPWL and the best snapshot (mod.rs:176-199,
+ *      common.rs:319-374, 485-495, wirelength.rs:10-51) ---- */
+typedef struct orc_legal {
+    int32_t n_bels, n_children;
+    const int32_t* bel_x;
+    const int32_t* bel_y;
+    const int32_t* bel_bucket;
+    const uint8_t* bel_taken;     /* NULL: none bound on entry */
+    const int32_t* cell_bucket;
+    const int32_t* child_root;    /* ascending; a root's children in constraint order */
+    const int32_t* child_dx;
+    const int32_t* child_dy;
+    const int32_t* child_bucket;
+    const int32_t* child_fixed;   /* fixed-cell index of a child with pins, or -1 */
+    const uint8_t* net_has_driver;/* NULL: every net has one */
+} orc_legal;
+int orc_legalize_full(const orc_design* d, const orc_legal* lp, const double* x, const double* y,
+                      uint8_t* taken, int32_t* cell_bel, int32_t* child_bel, double* displacement);
+double orc_legal_hpwl(const orc_design* d, const orc_legal* lp, const int32_t* cell_bel,
+                      const int32_t* child_bel);
+int orc_place_legalized(const orc_design* d, const orc_cfg* cfg, const orc_legal* lp,
+                        int32_t* cell_bel, int32_t* child_bel, double* best_hpwl, int* best_iter,
+                        int* legalizations, orc_result* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,13 @@ class OrcNesterov(C.Structure):
                 ("iter", C.c_int), ("a", C.c_double)]
 
 
+class OrcLegal(C.Structure):
+    _fields_ = [("n_bels", C.c_int32), ("n_children", C.c_int32), ("bel_x", _p), ("bel_y", _p),
+                ("bel_bucket", _p), ("bel_taken", _p), ("cell_bucket", _p), ("child_root", _p),
+                ("child_dx", _p), ("child_dy", _p), ("child_bucket", _p), ("child_fixed", _p),
+                ("net_has_driver", _p)]
+
+
 _LIB = None
 
 
@@ -90,6 +97,13 @@ def lib() -> C.CDLL:
             "orc_rng_shuffle_u32": (None, [C.POINTER(C.c_uint64), _p, C.c_int]),
             "orc_legalize": (C.c_int, [C.c_int, _p, _p, _p, C.c_int, _p, _p, _p, _p, _p, _p]),
             "orc_total_hpwl_int": (d, [C.POINTER(OrcDesign), _p, _p]),
+            "orc_legalize_full": (C.c_int, [C.POINTER(OrcDesign), C.POINTER(OrcLegal), _p, _p, _p, _p,
+                                            _p, _p]),
+            "orc_legal_hpwl": (d, [C.POINTER(OrcDesign), C.POINTER(OrcLegal), _p, _p]),
+            "orc_place_legalized": (C.c_int, [C.POINTER(OrcDesign), C.POINTER(OrcCfg),
+                                              C.POINTER(OrcLegal), _p, _p, C.POINTER(d),
+                                              C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                              C.POINTER(OrcResult)]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -256,3 +270,57 @@ class Nesterov:
     @property
     def positions(self):
         return self.x[: self.s.n].copy()
+
+
+class Legal:
+    """Keeps the arrays of a legalization view (an eisenjoch_amd.LegalProblem)
+    alive behind an OrcLegal."""
+
+    def __init__(self, lp):
+        self.lp = lp
+        self.c = OrcLegal(len(lp.bel_x), lp.n_children, P(lp.bel_x), P(lp.bel_y), P(lp.bel_bucket),
+                          P(lp.bel_taken), P(lp.cell_bucket), P(lp.child_root), P(lp.child_dx),
+                          P(lp.child_dy), P(lp.child_bucket), P(lp.child_fixed), P(lp.net_has_driver))
+
+    @property
+    def ref(self):
+        return C.byref(self.c)
+
+
+def legalize_full(fd, lp, x, y):
+    """mod.rs:206-256 with cluster children, full bucket scans.  Returns
+    (rc, cell_bel, child_bel, displacement)."""
+    D, G = Design(fd), Legal(lp)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    taken = (np.zeros(len(lp.bel_x), np.uint8) if lp.bel_taken is None else lp.bel_taken.copy())
+    cb = np.full(fd.n_mov, -1, np.int32)
+    kb = np.full(lp.n_children, -1, np.int32)
+    disp = C.c_double()
+    rc = lib().orc_legalize_full(D.ref, G.ref, P(x), P(y), P(taken), P(cb), P(kb), C.byref(disp))
+    return rc, cb, kb, disp.value
+
+
+def legal_hpwl(fd, lp, cell_bel, child_bel):
+    D, G = Design(fd), Legal(lp)
+    cb = np.ascontiguousarray(cell_bel, np.int32)
+    kb = np.ascontiguousarray(child_bel, np.int32)
+    return lib().orc_legal_hpwl(D.ref, G.ref, P(cb), P(kb))
+
+
+def place_legalized(fd, lp, *, max_iters=300, wl_coeff=0.5, step=0.1, interval=5, min_iters=20,
+                    overflow_target=0.1, field_mode=0, stop_every_iter=0):
+    """place_electro end to end (continuous loop, legalization and best
+    snapshot at each check iteration, final legalize(best))."""
+    D, G = Design(fd), Legal(lp)
+    cfg = OrcCfg(1, wl_coeff, 1.0, step, max_iters, interval, stop_every_iter, min_iters,
+                 overflow_target, field_mode)
+    cb = np.full(fd.n_mov, -1, np.int32)
+    kb = np.full(lp.n_children, -1, np.int32)
+    best = C.c_double()
+    bi, nl = C.c_int(), C.c_int()
+    res = OrcResult()
+    rc = lib().orc_place_legalized(D.ref, C.byref(cfg), G.ref, P(cb), P(kb), C.byref(best), C.byref(bi),
+                                   C.byref(nl), C.byref(res))
+    return dict(rc=rc, cell_bel=cb, child_bel=kb, best_hpwl=best.value, best_iter=bi.value,
+                legalizations=nl.value, iters=res.iters, converged=bool(res.converged))
