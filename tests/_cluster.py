@@ -38,3 +38,37 @@ def clustered_config1(n_pairs: int = 60, seed: int = 3):
                          child_dx=np.zeros(n_pairs), child_dy=np.zeros(n_pairs),
                          child_bucket=np.ones(n_pairs), child_fixed=F0 + np.arange(n_pairs))
     return cd, lp
+
+
+def clustered_context(n_pairs: int = 40, seed: int = 5):
+    """The config-1 design as a placement context (ContextView): its cells
+    on their initial BELs, its IO cells locked on BELs of their own bucket
+    at the perimeter locations, and `n_pairs` LUT+FF clusters whose FF
+    children sit on free FF BELs and carry two nets each."""
+    from eisenjoch_amd.flatten import CellView, ChipView, ContextView, NetView
+    d = ep.config1()
+    rng = np.random.default_rng(seed)
+    nb = len(d.bel_x)
+    bel_x = np.concatenate([d.bel_x, d.fixed_x.astype(np.int32)])
+    bel_y = np.concatenate([d.bel_y, d.fixed_y.astype(np.int32)])
+    bel_b = np.concatenate([d.bel_type, np.full(d.n_fixed, 2, np.int32)])
+    chip = ChipView(d.W, d.H, bel_x, bel_y, bel_b)
+    cells = [CellView(int(t), bel=int(b)) for t, b in zip(d.cell_type, d.init_bel)]
+    cells += [CellView(2, bel=nb + j, locked=True) for j in range(d.n_fixed)]
+    used = set(int(b) for b in d.init_bel)
+    free_ff = [b for b in range(nb) if d.bel_type[b] == 1 and b not in used]
+    luts = np.nonzero(d.cell_type == 0)[0]
+    roots = np.sort(rng.choice(luts, n_pairs, replace=False))
+    clusters = {}
+    nets = []
+    for e in range(d.n_nets):
+        p = [int(c) for c in d.pin_cell[d.net_off[e]:d.net_off[e + 1]]]
+        nets.append(NetView(p[0], p[1:]))
+    for k, r in enumerate(roots):
+        ch = len(cells)
+        cells.append(CellView(1, bel=int(free_ff[k]), cluster=int(r)))
+        cells[int(r)].cluster = int(r)
+        clusters[int(r)] = [ch]
+        nets.append(NetView(ch, [int(r)]))
+        nets.append(NetView(int(rng.integers(0, d.n_mov)), [ch]))
+    return ContextView(chip, cells, nets, clusters)

@@ -87,3 +87,27 @@ def test_fast_best_snapshot_and_context(device):
     _valid(d, lp, ctx.cell_bel, np.zeros(0, np.int64))
     assert ctx.best["best_hpwl"] == ctx.total_hpwl()
     assert abs(ctx.total_hpwl() - o["best_hpwl"]) <= 0.03 * o["best_hpwl"]
+
+
+def test_flattened_context_places_like_the_oracle(device):
+    # the drop-in path end to end: a context flattened in reference order,
+    # placed with the best snapshot on the GPU (FP64), equals the oracle run
+    # on the same problem, and its binding writes back onto the context
+    from _cluster import clustered_context
+    from eisenjoch_amd.flatten import flatten_for_gpu
+    ctx = clustered_context()
+    f = flatten_for_gpu(ctx)
+    cfg = ep.ElectroPlaceCfg(max_iters=100, field_mode="idxst")
+    i = device.upload(f.design, "fp64")
+    try:
+        log_case("flattened context, ep_place_legalized fp64")
+        r, cb, kb, info = device.place_legalized(i, cfg, f.legal)
+    finally:
+        device.free(i)
+    o = O.place_legalized(f.design, f.legal, max_iters=100, field_mode=1)
+    assert o["rc"] == 0 and info["best_iter"] == o["best_iter"]
+    np.testing.assert_array_equal(cb, o["cell_bel"])
+    np.testing.assert_array_equal(kb, o["child_bel"])
+    f.write_back(ctx, cb, kb)
+    bels = [c.bel for c in ctx.cells if c.alive]
+    assert min(bels) >= 0 and len(set(bels)) == len(bels)
