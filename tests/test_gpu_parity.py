@@ -273,6 +273,42 @@ def test_trajectory_fp64_matches_oracle(device, mode):
     np.testing.assert_allclose(r.x, o["x"], atol=1e-5 * d.W)
 
 
+@pytest.mark.parametrize("n,seed", [(5000, 3), (20000, 21)])
+def test_trajectory_fp64_whole_run_idxst(device, n, seed):
+    # the whole IDXST run of a batch-rule design (70-90 iterations): same
+    # iteration count, every iteration's overflow, penalty and step sizes
+    # within 1e-3 relative (measured: 1e-4 at most, growing from 1e-14 in
+    # the first 20 iterations), final HPWL within 1e-5
+    d = ep.sized_design(n, seed)
+    log_case(f"whole trajectory fp64 idxst n={n}")
+    i = device.upload(d, "fp64")
+    try:
+        r = device.place(i, ep.ElectroPlaceCfg(max_iters=300, field_mode="idxst"), log=True)
+    finally:
+        device.free(i)
+    o = O.place(d, max_iters=300, field_mode=1)
+    assert r.iters == o["iters"] and r.converged and o["converged"]
+    _compare_logs(r.log, o["log"], 20, 1e-9, "fp64 first 20")
+    _compare_logs(r.log, o["log"], r.iters, 1e-3, "fp64 whole run")
+    assert r.hpwl == pytest.approx(o["hpwl"], rel=1e-5)
+
+
+def test_trajectory_fast_reference_field(device):
+    # the reference field is chaotic: rounding differences grow about
+    # tenfold every two iterations (FP64: 3e-13 by iteration 20, 3e-8 by 40,
+    # O(1) by 80; FAST: 7e-4 by 20).  Its first 20 FAST iterations follow
+    # the oracle within 2e-3
+    d = ep.config1()
+    log_case("trajectory fast reference field")
+    i = device.upload(d, "fast")
+    try:
+        r = device.place(i, ep.ElectroPlaceCfg(max_iters=60), log=True)
+    finally:
+        device.free(i)
+    o = O.place(d, max_iters=60)
+    _compare_logs(r.log, o["log"], 20, 2e-3, "fast reference")
+
+
 def test_trajectory_fast_tracks_oracle(device):
     # the FAST path follows the FP64 oracle's early trajectory and reaches the
     # overflow target within one check interval of it, with HPWL within 0.5 %
@@ -285,9 +321,24 @@ def test_trajectory_fast_tracks_oracle(device):
     finally:
         device.free(i)
     o = O.place(d, max_iters=300, field_mode=1)
-    _compare_logs(r.log, o["log"], 5, 1e-3, "fast")
+    # the whole run (21 iterations) within 2e-4 (measured 5e-5)
+    _compare_logs(r.log, o["log"], min(r.iters, o["iters"]), 2e-4, "fast")
     assert r.converged and o["converged"]
     assert abs(r.iters - o["iters"]) <= 10
+    assert r.hpwl == pytest.approx(o["hpwl"], rel=5e-3)
+    # a longer run (a 20k-cell design, about 90 iterations): the first 20
+    # iterations within 1e-4 (measured 9e-6), then FP32 rounding makes the
+    # runs drift apart by a few percent; they stop within two iterations of
+    # each other and their HPWL agrees within 0.5 %
+    d = ep.sized_design(20000, 21)
+    i = device.upload(d, "fast")
+    try:
+        r = device.place(i, cfg, log=True)
+    finally:
+        device.free(i)
+    o = O.place(d, max_iters=300, field_mode=1)
+    _compare_logs(r.log, o["log"], 20, 1e-4, "fast 20k")
+    assert abs(r.iters - o["iters"]) <= 2
     assert r.hpwl == pytest.approx(o["hpwl"], rel=5e-3)
 
 
