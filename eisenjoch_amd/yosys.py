@@ -59,8 +59,8 @@ def parse_bit_value(v):
             return ONE
         if v in ("x", "z"):
             return UNDEF
-        raise NetlistError(f"Unknown constant bit value: {v!r}")
-    raise NetlistError(f"Invalid bit value in connections array: {v!r}")
+        raise NetlistError(f"constant bit {v!r} is not one of 0, 1, x, z")
+    raise NetlistError(f"a connection bit must be a net index or a constant string, got {v!r}")
 
 
 def infer_port_direction(cell_type: str, port: str) -> str:
@@ -81,7 +81,7 @@ def parse_port_direction(d: str) -> str:
     try:
         return {"input": IN, "output": OUT, "inout": INOUT}[d]
     except (KeyError, TypeError):
-        raise NetlistError(f"Unknown port direction: {d!r}") from None
+        raise NetlistError(f"port direction {d!r} is not input, output or inout") from None
 
 
 @dataclass(frozen=True)
@@ -128,7 +128,7 @@ def parse_property(v) -> Prop:
         if v and all(c in "01" for c in v):
             return Prop("bits", v)
         return Prop("str", v)
-    raise NetlistError(f"Unsupported property value type: {v!r}")
+    raise NetlistError(f"property values must be strings or integers, got {v!r}")
 
 
 def port_bit_name(base: str, i: int, total: int) -> str:
@@ -252,7 +252,7 @@ def _find_top(modules: dict) -> Tuple[str, dict]:
     """mod.rs:119-143: the first module (in key order) whose `top`
     attribute is a nonzero integer, else the first module."""
     if not modules:
-        raise NetlistError("No modules found in JSON")
+        raise NetlistError("the netlist defines no module")
     for name, mod in _sorted(modules):
         attrs = mod.get("attributes") if isinstance(mod, dict) else None
         if isinstance(attrs, dict) and "top" in attrs:
@@ -271,7 +271,7 @@ def _net_of(bit, bit_to_net: Dict[int, int], gnd: int, vcc: int) -> Optional[int
     if bit is UNDEF:
         return None
     if bit not in bit_to_net:
-        raise NetlistError("Signal bit index not found in net map")
+        raise NetlistError("a connection names a bit that no net carries")
     return bit_to_net[bit]
 
 
@@ -280,9 +280,9 @@ def parse_json(text: str) -> Netlist:
     try:
         root = json.loads(text)
     except (ValueError, TypeError) as e:
-        raise NetlistError(f"Failed to parse JSON: {e}") from e
+        raise NetlistError(f"the netlist is not valid JSON: {e}") from e
     if not isinstance(root, dict) or not isinstance(root.get("modules"), dict):
-        raise NetlistError("Missing or invalid 'modules' key in JSON")
+        raise NetlistError("the netlist has no 'modules' object")
     top_name, mod = _find_top(root["modules"])
     nl = Netlist(top_module=top_name)
     mod = mod if isinstance(mod, dict) else {}
@@ -294,7 +294,7 @@ def parse_json(text: str) -> Netlist:
 
     def see(bits, what):
         if not isinstance(bits, list):
-            raise NetlistError(f"{what} bits must be an array")
+            raise NetlistError(f"the bits of {what} are not a list")
         for b in bits:
             v = parse_bit_value(b)
             if _is_int(v) and v not in bit_to_net:
@@ -322,18 +322,18 @@ def parse_json(text: str) -> Netlist:
         cj = _obj(cj, f"cell {cname!r}")
         ctype = cj.get("type")
         if not isinstance(ctype, str):
-            raise NetlistError(f"Cell {cname!r} missing 'type' field")
+            raise NetlistError(f"cell {cname!r} has no 'type'")
         ci = nl.add_cell(cname, ctype)
         dirs = {}
         if isinstance(cj.get("port_directions"), dict):
             for pname, dv in cj["port_directions"].items():
                 if not isinstance(dv, str):
-                    raise NetlistError("port_direction value must be a string")
+                    raise NetlistError("a cell's port direction is not a string")
                 dirs[pname] = parse_port_direction(dv)
         if isinstance(cj.get("connections"), dict):
             for pname, bits in _sorted(cj["connections"]):
                 if not isinstance(bits, list):
-                    raise NetlistError("Connection bits must be an array")
+                    raise NetlistError("a cell connection's bits are not a list")
                 ptype = dirs.get(pname) or infer_port_direction(ctype, pname)
                 for i, b in enumerate(bits):
                     bit = parse_bit_value(b)
@@ -353,11 +353,11 @@ def parse_json(text: str) -> Netlist:
     for pname, pj in _sorted(ports):
         pj = _obj(pj, f"port {pname!r}")
         if not isinstance(pj.get("direction"), str):
-            raise NetlistError(f"Top-level port {pname!r} missing 'direction'")
+            raise NetlistError(f"top-level port {pname!r} has no 'direction'")
         pdir = parse_port_direction(pj["direction"])
         bits = pj.get("bits")
         if not isinstance(bits, list):
-            raise NetlistError(f"Top-level port {pname!r} missing 'bits'")
+            raise NetlistError(f"top-level port {pname!r} has no 'bits'")
         ctype, iport, itype = io[pdir]
         for i, b in enumerate(bits):
             bit = parse_bit_value(b)
@@ -376,7 +376,7 @@ def parse_json(text: str) -> Netlist:
             continue
         bits = nj.get("bits")
         if not isinstance(bits, list):
-            raise NetlistError("netnames entry missing 'bits'")
+            raise NetlistError("a netnames entry has no 'bits'")
         for i, b in enumerate(bits):
             v = parse_bit_value(b)
             if _is_int(v) and v in bit_to_net:

@@ -232,6 +232,30 @@ def test_density_gradient_parity(device, designs, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fast"])
+def test_density_gradient_linear_field(device, designs, prec):
+    # a field linear in the bin indices: for interior unit cells on bins =
+    # tiles the overlap weights average the bin index to centre - 1/2, so the
+    # gradient is the field at (x - 1/2, y - 1/2); distinct coefficients per
+    # axis and component catch an index, axis or component mix-up
+    d = designs["config1"]
+    log_case(f"density_grad linear field {prec}")
+    i = device.upload(d, prec)
+    try:
+        x, y = rand_pos(d, 12, edges=False)
+        x = np.clip(x, 1.0, d.W - 2.0)
+        y = np.clip(y, 1.0, d.H - 2.0)
+        by_, bx_ = np.mgrid[0:d.By, 0:d.Bx].astype(np.float64)
+        ex = 0.25 * bx_ - 0.375 * by_ + 1.0  # dyadic: exact in FP32
+        ey = -0.125 * bx_ + 0.5 * by_ + 0.5
+        gx, gy = device.density_grad(i, x, y, ex, ey)
+        tol = 1e-13 if prec == "fp64" else 1e-6
+        np.testing.assert_allclose(gx, 0.25 * (x - 0.5) - 0.375 * (y - 0.5) + 1.0, rtol=tol, atol=tol)
+        np.testing.assert_allclose(gy, -0.125 * (x - 0.5) + 0.5 * (y - 0.5) + 0.5, rtol=tol, atol=tol)
+    finally:
+        device.free(i)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fast"])
 def test_hpwl_parity(device, designs, prec):
     d = designs["small_hf"]
     log_case(f"hpwl {prec}")

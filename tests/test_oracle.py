@@ -313,6 +313,24 @@ def test_gather_constant_field():
     np.testing.assert_allclose(gy, -1.25, rtol=1e-14)
 
 
+def test_gather_linear_field_closed_form():
+    # a field linear in the bin indices, E = alpha bx + beta by + gamma: over
+    # a footprint of integer side the overlap weights average floor(u) to
+    # (centre - 1/2) exactly, so the gathered gradient is the field at
+    # (x - 1/2, y - 1/2).  Different coefficients per component and axis
+    # make a bin-index, axis or component mix-up fail.
+    rng = np.random.default_rng(16)
+    x = rng.uniform(1.5, 5.5, 40); y = rng.uniform(1.5, 5.5, 40)
+    x[:4] = [2.0, 3.5, 4.25, 1.5]  # a split 50/50, a whole bin, dyadic, an edge
+    d = tiny_design(x, y)
+    by, bx = np.mgrid[0:8, 0:8].astype(np.float64)
+    ex = 2.0 * bx - 3.0 * by + 1.0
+    ey = -1.0 * bx + 5.0 * by + 0.5
+    gx, gy = O.density_gradient(d, x, y, ex, ey)
+    np.testing.assert_allclose(gx, 2.0 * (x - 0.5) - 3.0 * (y - 0.5) + 1.0, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(gy, -1.0 * (x - 0.5) + 5.0 * (y - 0.5) + 0.5, rtol=0, atol=1e-13)
+
+
 # -------------------------------------------------------------- Nesterov --
 
 def test_fista_closed_form():
@@ -403,6 +421,24 @@ def test_xorshift_known_answer():
     st = __import__("ctypes").c_uint64(7)
     O.lib().orc_rng_shuffle_u32(__import__("ctypes").byref(st), O.P(a), 10)
     assert sorted(a.tolist()) == list(range(10))
+    # the next outputs from seed 1 (s ^= s << 13; s ^= s >> 7; s ^= s << 17)
+    st = __import__("ctypes").c_uint64(1)
+    seq = [O.lib().orc_rng_next(__import__("ctypes").byref(st)) for _ in range(4)]
+    assert seq == [1082269761, 1152992998833853505, 11177516664432764457, 17678023832001937445]
+
+
+def test_shuffle_known_answer():
+    # Fisher-Yates from the top with next_range = (u64 as u32) % (i + 1)
+    # (rng.rs:45-47, 52-61), seed 1, on [0, 1, 2, 3, 4], worked by hand:
+    #   i = 4: low word 1082269761 % 5 = 1 -> swap 4, 1 -> [0, 4, 2, 3, 1]
+    #   i = 3: low word  201397313 % 4 = 1 -> swap 3, 1 -> [0, 3, 2, 4, 1]
+    #   i = 2: low word 1854285353 % 3 = 2 -> no change
+    #   i = 1: low word 1432191013 % 2 = 1 -> no change
+    import ctypes
+    a = np.arange(5, dtype=np.uint32)
+    st = ctypes.c_uint64(1)
+    O.lib().orc_rng_shuffle_u32(ctypes.byref(st), O.P(a), 5)
+    assert a.tolist() == [0, 3, 2, 4, 1]
 
 
 # ---------------------------------------------------------------- metrics --
