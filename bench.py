@@ -239,7 +239,8 @@ def serve(args):
                    "stop_check": "every iteration" if args.stop == "every" else "every 5 iterations"},
         "dtype": "f32+bf16x3-mfma+f64-reductions", "data": "synthetic (seeded generator)",
         "serving_stats": {**st, "batch_builds_per_s": round(st["batch_builds"] / wall, 2),
-                          "graph_captures_per_s": round(st["graph_captures"] / wall, 2)},
+                          "graph_captures_per_s": round(st["graph_captures"] / wall, 2),
+                          "graph_updates_per_s": round(st["graph_updates"] / wall, 2)},
     }
     # HPWL of every served result against the plain FP64 placer on the same
     # pool design (one design at a time, after the serving window)

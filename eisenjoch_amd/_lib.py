@@ -95,7 +95,8 @@ class EpServeConfig(C.Structure):
 
 class EpServeStats(C.Structure):
     _fields_ = [("admitted", C.c_uint64), ("finished", C.c_uint64), ("batch_builds", C.c_uint64),
-                ("graph_captures", C.c_uint64), ("chunks", C.c_uint64), ("failed", C.c_int32),
+                ("graph_captures", C.c_uint64), ("graph_updates", C.c_uint64), ("chunks", C.c_uint64),
+                ("failed", C.c_int32),
                 ("reserved", C.c_int32)]
 
 

@@ -70,7 +70,21 @@ int engine_for(Ctx* c, int prec, int Bx, int By) {
 // ext_cap bytes (EP_ERR_INVALID_ARG when it does not fit; nothing is allocated).
 int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<const ep_config*>& cfgs,
                      Batch& b, void* ext_mem, size_t ext_cap) {
+    // a batch that keeps its graphs hands them to its next membership, stale
+    hipGraphExec_t kept[2] = {nullptr, nullptr};
+    int kept_iters[2] = {0, 0};
+    if (b.keep_graphs)
+        for (int k = 0; k < 2; k++) {
+            kept[k] = b.graph[k];
+            kept_iters[k] = b.graph_iters[k];
+            b.graph[k] = nullptr;
+        }
     batch_release(b);
+    for (int k = 0; k < 2; k++) {
+        b.graph[k] = kept[k];
+        b.graph_iters[k] = kept_iters[k];
+        b.graph_stale[k] = kept[k] != nullptr;
+    }
     b.ds = ds;
     const int nd = (int)ds.size();
     if (nd == 0) return EP_OK;
@@ -471,6 +485,46 @@ static bool batch_needs_fix(const Ctx* c, const Batch& b) {
     return false;
 }
 
+// Makes the batch's graph of `iters` iterations (with or without the fix
+// pass) current: captured and instantiated, or, when a kept graph of the
+// same launches exists, captured and updated in place.  Launches nothing, so
+// a caller can prepare the next chunk's graph while the GPU runs this one.
+int batch_graph_ready(Ctx* c, Batch& b, int iters, bool with_fix) {
+    const int k = with_fix ? 1 : 0;
+    hipGraphExec_t& ge = b.graph[k];
+    int& gi = b.graph_iters[k];
+    if (ge && gi == iters && !b.graph_stale[k]) return EP_OK;
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < iters; i++) enqueue_iteration(c, b, with_fix);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    bool updated = false;
+    if (ge && gi == iters && b.graph_stale[k]) {
+        // same launches, new grids and arguments: update the instantiated graph
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        updated = hipGraphExecUpdate(ge, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (!updated) (void)hipGetLastError();  // the launches changed: instantiate afresh
+    }
+    if (updated) {
+        c->graph_updates++;
+    } else {
+        if (ge) hipGraphExecDestroy(ge);
+        ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(g);
+            return fail(c, EP_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        c->graph_captures++;
+    }
+    hipGraphDestroy(g);
+    gi = iters;
+    b.graph_stale[k] = false;
+    return EP_OK;
+}
+
 int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph, int fix) {
     if (b.ds.empty() || iters <= 0) return EP_OK;
     const bool with_fix = fix < 0 ? batch_needs_fix(c, b) : fix != 0;
@@ -479,23 +533,9 @@ int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph, int fix) {
         HIPCHK(c, hipGetLastError());
         return EP_OK;
     }
-    hipGraphExec_t& ge = b.graph[with_fix ? 1 : 0];
-    int& gi = b.graph_iters[with_fix ? 1 : 0];
-    if (!ge || gi != iters) {
-        if (ge) hipGraphExecDestroy(ge);
-        ge = nullptr;
-        hipGraph_t g = nullptr;
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < iters; i++) enqueue_iteration(c, b, with_fix);
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        if (e != hipSuccess) return fail(c, EP_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
-        gi = iters;
-        c->graph_captures++;
-    }
-    HIPCHK(c, hipGraphLaunch(ge, c->stream));
+    int rc = batch_graph_ready(c, b, iters, with_fix);
+    if (rc) return rc;
+    HIPCHK(c, hipGraphLaunch(b.graph[with_fix ? 1 : 0], c->stream));
     return EP_OK;
 }
 

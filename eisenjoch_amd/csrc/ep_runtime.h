@@ -78,6 +78,11 @@ struct Batch {
     BatchDev bd{};
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // replayed chunk without / with the fix pass
     int graph_iters[2] = {0, 0};
+    // keep_graphs: a rebuild keeps the instantiated graphs, marked stale; the
+    // next launch captures the new chunk and updates them in place
+    // (hipGraphExecUpdate), instantiating only when the launches changed
+    bool keep_graphs = false;
+    bool graph_stale[2] = {false, false};
     std::vector<double> done_ms;     // per design: device time from its reset to its stop
 };
 
@@ -114,6 +119,7 @@ struct Ctx : ep_ctx {
     ep_config eval_cfg{};            // configuration of the ep_eval_* calls
     bool server_running = false;
     int graph_captures = 0;          // hipGraphs instantiated by batch_run_iters
+    int graph_updates = 0;           // instantiated hipGraphs updated in place instead
     std::map<std::pair<int, int>, int> tune;   // (Bx, By) -> transform engine (autotuned)
     std::unique_ptr<Server, ServerDeleter> server;
 };
@@ -189,6 +195,9 @@ int batch_reset(Ctx* c, Batch& b, const ep_config* cfg);
 // fix: 1 / 0 launches the iterations with / without the fix pass; -1 decides
 // from the host mirror of the states.
 int batch_run_iters(Ctx* c, Batch& b, int iters, bool use_graph, int fix = -1);
+// Captures (or updates in place) the batch's graph of `iters` iterations
+// without launching it.
+int batch_graph_ready(Ctx* c, Batch& b, int iters, bool with_fix);
 int batch_finalize(Ctx* c, Batch& b);
 int read_states(Ctx* c, const Batch& b);
 void fill_result(const Design& D, const DevState& s, ep_result* r);

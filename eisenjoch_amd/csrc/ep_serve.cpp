@@ -195,6 +195,7 @@ static void serve_loop(Ctx* c) {
         }
         if (S.active.empty()) continue;
         if (S.dirty) {
+            S.batch.keep_graphs = getenv("EP_SERVE_NO_GRAPH_UPDATE") == nullptr;
             std::vector<Design*> ds;
             std::vector<const ep_config*> cf;
             for (ServeReq* r : S.active) {
@@ -216,9 +217,19 @@ static void serve_loop(Ctx* c) {
             rc = fail(c, EP_ERR_DEVICE, "forced serving failure (EP_SERVE_FAIL_AFTER)");
             break;
         }
-        const int caps = c->graph_captures;
-        if ((rc = batch_run_iters(c, S.batch, S.scfg.iters_per_sync, getenv("EP_NO_GRAPH") == nullptr))) break;
+        const int caps = c->graph_captures, ups = c->graph_updates;
+        const bool graphs = getenv("EP_NO_GRAPH") == nullptr;
+        if (graphs && S.batch.graph_stale[0] && S.batch.keep_graphs) {
+            // the first chunk of a new membership runs eagerly, and the graph of
+            // the next chunks (new designs have their penalty by then) is
+            // recorded and updated while the GPU runs it
+            if ((rc = batch_run_iters(c, S.batch, S.scfg.iters_per_sync, false))) break;
+            if ((rc = batch_graph_ready(c, S.batch, S.scfg.iters_per_sync, false))) break;
+        } else if ((rc = batch_run_iters(c, S.batch, S.scfg.iters_per_sync, graphs))) {
+            break;
+        }
         S.stats.graph_captures += (uint64_t)(c->graph_captures - caps);
+        S.stats.graph_updates += (uint64_t)(c->graph_updates - ups);
         S.stats.chunks++;
         if ((rc = read_states(c, S.batch))) break;
         bool changed = false;
@@ -256,7 +267,17 @@ static void serve_loop(Ctx* c) {
             std::lock_guard<std::mutex> lk(S.m);
             S.active.erase(std::remove(S.active.begin(), S.active.end(), nullptr), S.active.end());
             S.dirty = true;
-            if (S.active.empty()) batch_release(S.batch);
+            if (S.active.empty()) {
+                // nothing left to run: free the launch descriptors; the graphs
+                // stay (stale) for the next admission to update in place
+                if (S.batch.keep_graphs) {
+                    std::vector<Design*> none;
+                    std::vector<const ep_config*> nocfg;
+                    batch_build_cfgs(c, none, nocfg, S.batch, nullptr, 0);
+                } else {
+                    batch_release(S.batch);
+                }
+            }
         }
     }
     std::lock_guard<std::recursive_mutex> g(c->mu);

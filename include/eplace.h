@@ -207,6 +207,7 @@ typedef struct ep_serve_stats {
     uint64_t finished;        /* requests that left it (done or cancelled) */
     uint64_t batch_builds;    /* batch rebuilds after membership changes */
     uint64_t graph_captures;  /* iteration graphs captured and instantiated */
+    uint64_t graph_updates;   /* instantiated graphs updated in place after a rebuild */
     uint64_t chunks;          /* chunks of iterations launched */
     int32_t failed;           /* 0, or the status that stopped the serving thread */
     int32_t reserved;
