@@ -173,6 +173,23 @@ struct PlaneSum {
     int32_t blocks;   // blocks of 256 threads covering n
 };
 
+// One batch of row transforms of the FFT engine (FAST path, power-of-two
+// row lengths): rows x n input, row-major; element k of row r goes to
+// out[(k * ld + r) * ostride] (the plane transposed; ostride 2 writes one
+// component of an interleaved pair).  kind: 0 DCT-II, 1 DCT-III with the
+// k = 0 term halved (C3), 2 DST-III of the reference field (S3), 3 the IDXST
+// sine transform.  epi: 0 store, 1 the Poisson solve into out (Ex^) and out2
+// (Ey^), at (ky = k, kx = r).  A block transforms rpb rows.
+struct FftJob {
+    const void* in;
+    void* out;
+    void* out2;
+    const void* tw;      // float2[n]: exp(-2 pi i t / n)
+    const void* dtw;     // float2[n]: exp(-i pi k / 2n)
+    const DevState* st;  // a block of a done design returns at once
+    int32_t n, rows, rpb, kind, epi, ld, ostride, design;
+};
+
 // kEpiStore2: row-major into every other element (one component of a pair)
 enum Epilogue : int32_t { kEpiStore = 0, kEpiStoreT = 1, kEpiPoisson = 2, kEpiStore2 = 3 };
 
@@ -223,5 +240,15 @@ void launch_plane_sum(const PlaneSum* sums, int n_sums, int max_blocks, hipStrea
 // bjob: (job, tile) of each block of the grid (total_tiles pairs; -1: none).
 void launch_gemm_pass(const GemmJob* jobs_dev, const int32_t* bjob, int total_tiles,
                       const DesignDev* dd, int prec, int engine, int pass, hipStream_t s);
+
+// ---- transforms by FFT (ep_fft.hip) ----
+// Row lengths the FFT engine takes: powers of two from 16 to 512.
+inline bool fft_len_ok(int n) { return n >= 16 && n <= 512 && (n & (n - 1)) == 0; }
+// Rows one block of the FFT pass transforms (256 threads, 4 to 32 threads
+// per row).
+int fft_rows_per_block(int n);
+// fblk: (job, first row) of each block of the grid (blocks pairs; -1: none).
+void launch_fft_pass(const FftJob* jobs_dev, const int32_t* fblk, int blocks, const DesignDev* dd,
+                     hipStream_t s);
 
 }  // namespace ep

@@ -53,15 +53,24 @@ int validate_config(Ctx* c, const ep_config* cfg, int log_cap) {
     return EP_OK;
 }
 
-// Transform engine of a design shape: the autotuned choice when there is
-// one, else MFMA with LDS staging in K stages of 32 for BF16-eligible shapes
-// (bins multiples of 64 on the FAST path) and the FMA kernel otherwise.
+// Transform engine of a design shape: the autotuned (or pinned) choice when
+// there is one, else MFMA with LDS staging in K stages of 32 for BF16-eligible
+// shapes (bins multiples of 64 on the FAST path) and the FMA kernel otherwise.
+// The FP64 path always takes the FMA kernel.
 int engine_for(Ctx* c, int prec, int Bx, int By) {
-    const bool mfma_ok = prec == EP_PREC_FAST && Bx % 64 == 0 && By % 64 == 0 &&
-                         getenv("EP_FORCE_FMA") == nullptr;
-    if (!mfma_ok) return 1;
+    if (prec != EP_PREC_FAST || getenv("EP_FORCE_FMA") != nullptr) return 1;
     auto it = c->tune.find({Bx, By});
-    return it == c->tune.end() ? 0 : it->second;
+    if (it != c->tune.end()) return it->second;
+    return (Bx % 64 == 0 && By % 64 == 0) ? 0 : 1;
+}
+
+// Engines a FAST design shape can take.
+static std::vector<int> engines_for_shape(int Bx, int By) {
+    std::vector<int> e;
+    if (Bx % 64 == 0 && By % 64 == 0) e = {0, 3, 4, 2};
+    e.push_back(1);
+    if (fft_len_ok(Bx) && fft_len_ok(By)) e.push_back(kEngineFft);
+    return e;
 }
 
 // Per pass: (A kind, A matrix or buffer, B kind, B matrix or buffer, epilogue, out, out2).
@@ -99,6 +108,7 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
     pc[0] = pb[0] = pw[0] = ph[0] = ps[0] = pt[0] = 0;
     // jobs: [pass][engine]
     std::vector<GemmJob> jobs[4][kEngines];
+    std::vector<FftJob> fjobs[4];  // the FFT engine's row transforms, per pass
     std::vector<PlaneSum> sums;  // split final passes: planes summed after the transforms
     for (int i = 0; i < nd; i++) {
         Design& D = *ds[i];
@@ -149,6 +159,27 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         GemmJob i2b = mk(By, Bx, By, kEpiStore2, (char*)D.dev.field + tsz, nullptr);  // S3 . U (idst_cols)
         cst(i2b, true, ty, 2); dat(i2b, false, D.dev.uT[1]);
         const int eng = engine_for(c, b.prec, Bx, By);
+        if (eng == kEngineFft) {
+            // the six products as row transforms of the same planes
+            const int sk = cfgs[i]->field_mode == EP_FIELD_REFERENCE ? 2 : 3;
+            auto fj = [&](const void* in, void* out, void* out2, const Transform* t, int n, int rows,
+                          int kind, int epi, int ld, int os) {
+                FftJob F;
+                memset(&F, 0, sizeof(F));
+                F.in = in; F.out = out; F.out2 = out2;
+                F.tw = t->fft_tw; F.dtw = t->dct_tw; F.st = D.dev.st;
+                F.n = n; F.rows = rows; F.rpb = std::min(rows, fft_rows_per_block(n));
+                F.kind = kind; F.epi = epi; F.ld = ld; F.ostride = os; F.design = i;
+                return F;
+            };
+            fjobs[0].push_back(fj(D.dev.dens, D.dev.tT, nullptr, tx, Bx, By, 0, 0, By, 1));
+            fjobs[1].push_back(fj(D.dev.tT, D.dev.ehat[0], D.dev.ehat[1], ty, By, Bx, 0, 1, Bx, 1));
+            fjobs[2].push_back(fj(D.dev.ehat[0], D.dev.uT[0], nullptr, tx, Bx, By, sk, 0, By, 1));
+            fjobs[2].push_back(fj(D.dev.ehat[1], D.dev.uT[1], nullptr, tx, Bx, By, 1, 0, By, 1));
+            fjobs[3].push_back(fj(D.dev.uT[0], D.dev.field, nullptr, ty, By, Bx, 1, 0, Bx, 2));
+            fjobs[3].push_back(fj(D.dev.uT[1], (char*)D.dev.field + tsz, nullptr, ty, By, Bx, sk, 0, Bx, 2));
+            continue;
+        }
         if (eng == 4) {
             // split the long products along K (planes of partial sums); the
             // next pass sums a data operand's planes as it loads it
@@ -265,6 +296,19 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             bytes += align_up(sizeof(GemmJob) * std::max<size_t>(1, jobs[p][e].size()));
             bytes += align_up(sizeof(int32_t) * std::max<size_t>(2, bjob[p][e].size()));
         }
+    // FFT engine: (job, first row) of every block of a pass
+    std::vector<int32_t> fblk[4];
+    for (int p = 0; p < 4; p++) {
+        for (size_t q = 0; q < fjobs[p].size(); q++)
+            for (int r = 0; r < fjobs[p][q].rows; r += fjobs[p][q].rpb) {
+                fblk[p].push_back((int32_t)q);
+                fblk[p].push_back(r);
+            }
+        b.nfft[p] = (int)fjobs[p].size();
+        b.fft_blocks[p] = (int)(fblk[p].size() / 2);
+        bytes += align_up(sizeof(FftJob) * std::max<size_t>(1, fjobs[p].size()));
+        bytes += align_up(sizeof(int32_t) * std::max<size_t>(2, fblk[p].size()));
+    }
     bytes += align_up(sizeof(PlaneSum) * std::max<size_t>(1, sums.size()));
     // block -> design tables of the six grids
     const std::vector<int32_t>* pfxs[6] = {&pc, &pb, &pw, &ph, &ps, &pt};
@@ -319,6 +363,15 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             if (bjob[p][e].empty()) b.bjob2[p][e] = (int32_t*)put(none, sizeof(none));
             else b.bjob2[p][e] = (int32_t*)put(bjob[p][e].data(), sizeof(int32_t) * bjob[p][e].size());
         }
+    for (int p = 0; p < 4; p++) {
+        FftJob dummy;
+        memset(&dummy, 0, sizeof(dummy));
+        b.fjobs[p] = (FftJob*)(fjobs[p].empty() ? put(&dummy, sizeof(dummy))
+                                                : put(fjobs[p].data(), sizeof(FftJob) * fjobs[p].size()));
+        const int32_t none[2] = {-1, -1};
+        b.fblk[p] = (int32_t*)(fblk[p].empty() ? put(none, sizeof(none))
+                                               : put(fblk[p].data(), sizeof(int32_t) * fblk[p].size()));
+    }
     {
         PlaneSum none;
         memset(&none, 0, sizeof(none));
@@ -355,6 +408,7 @@ void batch_release(Batch& b) {
     b.mem = nullptr;
     b.ext_mem = false;
     b.ds.clear();
+    for (int p = 0; p < 4; p++) b.nfft[p] = b.fft_blocks[p] = 0;
 }
 
 static DevState fresh_state(const Design& D, const ep_config* cfg) {
@@ -407,6 +461,7 @@ static void enqueue_gemm(Ctx* c, Batch& b, int p) {
         if (b.njobs2[p][e] > 0)
             launch_gemm_pass(b.jobs2[p][e], b.bjob2[p][e], b.tiles2[p][e], b.d_dev, b.prec, e, p,
                              c->stream);
+    if (b.fft_blocks[p] > 0) launch_fft_pass(b.fjobs[p], b.fblk[p], b.fft_blocks[p], b.d_dev, c->stream);
     if (p == 3) launch_plane_sum(b.sums, b.n_sums, b.sum_blocks, c->stream);  // field = sum of planes
 }
 
@@ -827,7 +882,10 @@ extern "C" int ep_autotune(ep_ctx* h, const uint32_t* ids, uint32_t n, uint32_t 
         if (f) {
             int bx, by, e;
             while (fscanf(f, "%d %d %d", &bx, &by, &e) == 3)
-                if (e >= 0 && e < kEngines) cached[{bx, by}] = e;
+                if (e >= 0 && e < kNumEngines) {
+                    const std::vector<int> ok = engines_for_shape(bx, by);
+                    if (std::find(ok.begin(), ok.end(), e) != ok.end()) cached[{bx, by}] = e;
+                }
             fclose(f);
         }
     }
@@ -839,7 +897,7 @@ extern "C" int ep_autotune(ep_ctx* h, const uint32_t* ids, uint32_t n, uint32_t 
         if (!D) return fail(c, EP_ERR_NOT_FOUND, "unknown design id");
         if (D->prec != EP_PREC_FAST) continue;
         if (std::find(ds.begin(), ds.end(), D) == ds.end()) ds.push_back(D);
-        if (D->dev.Bx % 64 == 0 && D->dev.By % 64 == 0) shapes.insert({D->dev.Bx, D->dev.By});
+        if (engines_for_shape(D->dev.Bx, D->dev.By).size() > 1) shapes.insert({D->dev.Bx, D->dev.By});
     }
     std::vector<std::pair<int, int>> todo;
     for (auto& sh : shapes) {
@@ -852,13 +910,13 @@ extern "C" int ep_autotune(ep_ctx* h, const uint32_t* ids, uint32_t n, uint32_t 
         return (int64_t)a.first * a.second > (int64_t)b.first * b.second;
     });
     for (auto& sh : todo)
-        if (!c->tune.count(sh)) c->tune[sh] = 0;
+        if (!c->tune.count(sh)) c->tune[sh] = engine_for(c, EP_PREC_FAST, sh.first, sh.second);
     std::vector<std::pair<std::pair<int, int>, int>> fresh;
     for (auto& sh : todo) {
         float best = 1e30f;
         int best_e = c->tune[sh];
         const bool splittable = sh.first >= kSplitMinK || sh.second >= kSplitMinK;
-        for (int e : {0, 3, 4, 2, 1}) {
+        for (int e : engines_for_shape(sh.first, sh.second)) {
             if (e == 4 && !splittable) continue;  // same launches as engine 0
             c->tune[sh] = e;
             float ms = 0.f;
@@ -902,8 +960,10 @@ extern "C" int ep_tune_table(ep_ctx* h, int32_t* out, uint32_t cap, uint32_t* n)
 extern "C" int ep_tune_set(ep_ctx* h, int32_t bx, int32_t by, int32_t engine) {
     Ctx* c = (Ctx*)h;
     if (!c) return fail(c, EP_ERR_INVALID_ARG, "null context");
-    if (engine >= kEngines) return fail(c, EP_ERR_INVALID_ARG, "unknown transform engine");
-    if (engine >= 0 && engine != 1 && (bx % 64 || by % 64))
+    if (engine >= kNumEngines) return fail(c, EP_ERR_INVALID_ARG, "unknown transform engine");
+    if (engine == kEngineFft && !(fft_len_ok(bx) && fft_len_ok(by)))
+        return fail(c, EP_ERR_INVALID_ARG, "the FFT engine needs bin counts that are powers of two from 16 to 512");
+    if (engine >= 0 && engine != 1 && engine != kEngineFft && (bx % 64 || by % 64))
         return fail(c, EP_ERR_INVALID_ARG, "MFMA engines need bins that are multiples of 64");
     std::lock_guard<std::recursive_mutex> g(c->mu);
     if (c->server_running) return fail(c, EP_ERR_INVALID_ARG, "context is serving; stop it first");

@@ -21,8 +21,11 @@ constexpr int kStatePool = 8192;      // resident designs per context
 // Transform engines: 0 MFMA with LDS staging in K stages of 32, 1 FP32/FP64
 // FMA, 2 MFMA split-K inside a block, 3 MFMA with LDS staging in K stages of
 // 64, 4 engine 0 with products of K >= 512 split across blocks into K slices
-// of 128 (partial planes summed by the next pass).
-constexpr int kEngines = 5;
+// of 128 (partial planes summed by the next pass).  Engine 5 (kEngineFft)
+// runs every pass as batched row FFTs (FAST path, power-of-two bin counts).
+constexpr int kEngines = 5;      // dense-product engines (job lists per engine)
+constexpr int kEngineFft = 5;
+constexpr int kNumEngines = 6;
 constexpr int kDefaultLogCap = 2048;  // iteration records per design
 
 // Constant transform matrices of one size and field mode: [0] C2, [1] C3,
@@ -34,6 +37,8 @@ struct Transform {
     float* f[3] = {nullptr, nullptr, nullptr};
     void* hi[3] = {nullptr, nullptr, nullptr};
     void* lo[3] = {nullptr, nullptr, nullptr};
+    float* fft_tw = nullptr;  // float2[n]: exp(-2 pi i t / n)
+    float* dct_tw = nullptr;  // float2[n]: exp(-i pi k / 2n)
 };
 
 struct Design {
@@ -69,6 +74,9 @@ struct Batch {
     int tiles2[4][kEngines] = {};
     GemmJob* jobs2[4][kEngines] = {};
     int32_t* bjob2[4][kEngines] = {};  // job of each block of a pass (one load at block start)
+    int nfft[4] = {}, fft_blocks[4] = {};  // FFT engine: jobs and blocks of each pass
+    FftJob* fjobs[4] = {};
+    int32_t* fblk[4] = {};                 // (job, first row) of each block
     PlaneSum* sums = nullptr;        // split final passes (device), summed after pass 3
     int n_sums = 0, sum_blocks = 0;
     void* mem = nullptr;        // descriptors, prefixes, jobs

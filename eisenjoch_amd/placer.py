@@ -575,7 +575,7 @@ class Device:
                                             C.byref(k)), self._h, "ep_profile_iteration")
         return {out[i].name.decode(): out[i].avg_ms for i in range(k.value)}
 
-    ENGINES = {0: "mfma-k32", 1: "fma", 2: "mfma-splitk", 3: "mfma-k64", 4: "mfma-kslices"}
+    ENGINES = {0: "mfma-k32", 1: "fma", 2: "mfma-splitk", 3: "mfma-k64", 4: "mfma-kslices", 5: "fft"}
 
     def autotune(self, ids: Sequence[int], reps: int = 5, cache_path: Optional[str] = None) -> dict:
         """Pick the fastest transform engine per bin shape (ep_autotune);
