@@ -235,7 +235,8 @@ int ep_profile_iteration(ep_ctx* ctx, const uint32_t* ids, uint32_t n, const ep_
  * 1: FP32 FMA; 2: MFMA split-K inside a block; 3: MFMA with K stages of 64
  * through one LDS buffer;
  * 4: engine 0 with products of K >= 512 split across blocks into K slices
- * of 128) on the four
+ * of 128; 5: FP32 FFTs of the rows, bin counts that are powers of two from
+ * 16 to 512) on the four
  * transform passes of the whole batch `ids`, `reps` times, with the other
  * shapes at their current choice, and keeps the fastest per shape for every
  * later batch.  With a cache path, shapes found in the file are reused and
@@ -245,7 +246,8 @@ int ep_autotune(ep_ctx* ctx, const uint32_t* ids, uint32_t n, uint32_t reps, con
 int ep_tune_table(ep_ctx* ctx, int32_t* out, uint32_t cap, uint32_t* n);
 /* Pins the transform engine of one bin shape in the tuning table (engine < 0
  * removes the entry; the MFMA engines 0, 2, 3, 4 need bins that are multiples
- * of 64).  No counterpart in the reference. */
+ * of 64, the FFT engine 5 powers of two from 16 to 512).  No counterpart in
+ * the reference. */
 int ep_tune_set(ep_ctx* ctx, int32_t bx, int32_t by, int32_t engine);
 
 /* Copies `bytes` device-to-device once with the runtime's plain copy kernel
