@@ -1,12 +1,15 @@
 // Spectral Poisson solve by fast transforms (density.rs:96-166), FAST path,
 // bin counts that are powers of two.
 //
-// The passes compute what the dense-product passes of ep_gemm.hip compute:
-// F1 transforms the density's rows (DCT-II along x) and stores them
-// transposed, F2 transforms those rows (DCT-II along y) and applies the
-// Poisson solve, I1 transforms the rows of Ex^ and Ey^ (S3 and C3 along x)
-// and stores them transposed, I2 transforms those rows (C3 and S3 along y)
-// into the interleaved field.  Every 1-D transform of length n is one
+// Three passes compute the field the four dense-product passes of
+// ep_gemm.hip compute (Ex = C3_y (phi w_u) S3_x^T, Ey = S3_y (phi w_v) C3_x^T,
+// phi the Poisson-solved DCT-II coefficients; the inverse may take y first
+// because the transform is separable): a row pass transforms the density's
+// rows (DCT-II along x) and stores them transposed; the column solve takes
+// each of those rows through the DCT-II along y, the Poisson solve and both
+// inverse transforms along y, and stores the two results transposed; a row
+// pass per component finishes the inverse along x into the interleaved
+// field, in row order.  Every 1-D transform of length n is one
 // complex FFT of length n (Makhoul's reordering):
 //   DCT-II  v = (x0, x2, x4, ..., x5, x3, x1), X_k = Re(e^{-i pi k/2n} V_k)
 //   C3      W_k = e^{-i pi k/2n} (X_k + i X_{n-k}) with X_n = 0, w = FFT(W),
@@ -88,25 +91,153 @@ __device__ inline void dft_reg(f32x2 (&a)[N]) {
 }
 
 // Split of each supported length n = N1 N2 (N1 points per thread in the
-// first phase, N2 threads per row): 16 = 4 x 4, 32 = 8 x 4, 64 = 8 x 8,
-// 128 = 16 x 8, 256 = 16 x 16, 512 = 16 x 32.  A block of 256 threads takes
-// 256 / N2 rows.
+// first phase, N2 threads per row; a block of 256 threads takes 256 / N2
+// rows).  Row passes: 16 = 4 x 4, 32 = 8 x 4, 64 = 8 x 8, 128 = 16 x 8,
+// 256 = 16 x 16, 512 = 16 x 32.  The column solve keeps more in LDS per row
+// and takes 128 as 8 x 16.
 __host__ __device__ constexpr int fft_n2(int n) {
     return n >= 512 ? 32 : n >= 256 ? 16 : n >= 64 ? 8 : 4;
 }
+__host__ __device__ constexpr int fft_n2_solve(int n) {
+    return n >= 512 ? 32 : n >= 128 ? 16 : n >= 64 ? 8 : 4;
+}
 __host__ __device__ constexpr int fft_block_rows(int n) { return 256 / fft_n2(n); }
+__host__ __device__ constexpr int fft_solve_rows(int n) { return 256 / fft_n2_solve(n); }
 
-// LDS of the largest case (n = 128: 32 rows x 16 x (8 + 1) points)
+// LDS of the row pass (largest case: n = 128, 32 rows x 16 x (8 + 1) points)
 constexpr int kFftLds = 32 * 16 * 9;  // f32x2 elements
+// LDS of the column solve: the exchange (largest: n = 256, 16 x 16 x 17
+// points) and the Poisson-solved rows (largest: 16 rows x 257)
+constexpr int kSolveEx = 16 * 16 * 17;   // f32x2 elements
+constexpr int kSolveRows = 16 * 257;     // floats
 
+// The complex input points j = t + N2 m (m < N1) of thread t's share of one
+// row, for the transform kind: DCT-II takes v_j = x_{2j} (j < n/2) or
+// x_{2(n-1-j)+1}; the C3 kinds take e^{-i pi j/2n} (Z_j + i Z_{n-j}), Z_n = 0,
+// with Z the row (C3), the reversed row (DST-III) or (0, x_{n-1}, ..., x_1)
+// (IDXST).  row(i) returns element i of the row.  Every access is made for
+// every kind (indices wrapped into the row where the element is zeroed), so
+// no load's address depends on a branch over the kind.
+template <int N1, int N2, typename Row>
+__device__ inline void fft_input(int kind, int t, const EP_GLOBAL f32x2* dtw, Row row, f32x2 (&a)[N1]) {
+    constexpr int n = N1 * N2;
+#pragma unroll
+    for (int m = 0; m < N1; m++) {
+        const int j = t + N2 * m;
+        const int i0 = j < n / 2 ? 2 * j : 2 * (n - 1 - j) + 1;
+        const int ia = kind == 0 ? i0 : kind == 1 ? j : kind == 2 ? n - 1 - j : (n - j) & (n - 1);
+        const int ib = kind == 1 ? (n - j) & (n - 1) : kind == 2 ? (j - 1) & (n - 1) : j;
+        const float xa = row(ia), xb = row(ib);
+        const f32x2 c = dtw[j];
+        const float za = (kind == 3 && j == 0) ? 0.f : xa;
+        const float zb = j == 0 ? 0.f : xb;
+        const f32x2 w = cmul(c, (f32x2){za, zb});
+        a[m] = kind == 0 ? (f32x2){xa, 0.f} : w;
+    }
+}
+
+// Phase 1: the N1-point DFT of the thread's points, times e^{-2 pi i t k1/n},
+// into the exchange (row r, k1, t).
 template <int N1, int N2>
-__device__ inline void fft_rows(const FftJob& J, const DesignDev* dd, int r0, f32x2* ex) {
+__device__ inline void fft_phase1(f32x2 (&a)[N1], const EP_GLOBAL f32x2* tw, f32x2* ex, int r, int t) {
+    constexpr int PAD = N2 + 1;
+    dft_reg<N1>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) ex[(r * N1 + k1) * PAD + t] = cmul(a[k1], tw[t * k1]);
+}
+
+// Output index of phase-2 value (q, k2) of thread t: k = k1 + N1 k2 with
+// k1 = t + N2 q; DCT-II writes it at k, the C3 kinds at j = 2k (k < n/2) or
+// 2(n-1-k)+1.
+template <int N1, int N2>
+__device__ inline int fft_out_index(int kind, int t, int q, int k2) {
+    constexpr int n = N1 * N2;
+    const int k = t + N2 * q + N1 * k2;
+    return kind == 0 ? k : (k < n / 2 ? 2 * k : 2 * (n - 1 - k) + 1);
+}
+
+// Phase 2: for this thread's k1 = t + N2 q (< N1), the N2-point DFT of the
+// exchanged values; emit(q, k2, j, value) receives output k = k1 + N1 k2 of
+// the kind's real transform at its index j: DCT-II X_k = Re(e^{-i pi k/2n}
+// V_k); C3 kinds Re(w_k) / 2, signed (-1)^j for DST-III and -(-1)^j for IDXST.
+template <int N1, int N2, typename Emit>
+__device__ inline void fft_phase2(const f32x2* ex, int r, int t, int kind, const EP_GLOBAL f32x2* dtw,
+                                  Emit emit) {
+    constexpr int PAD = N2 + 1, NQ = (N1 + N2 - 1) / N2;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int k1 = t + N2 * q;
+        if (k1 < N1) {
+            f32x2 b[N2];
+#pragma unroll
+            for (int s2 = 0; s2 < N2; s2++) b[s2] = ex[(r * N1 + k1) * PAD + s2];
+            dft_reg<N2>(b);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; k2++) {
+                const int k = k1 + N1 * k2;
+                const f32x2 c = dtw[k];
+                const int j = fft_out_index<N1, N2>(kind, t, q, k2);
+                const float x0 = c.x * b[k2].x - c.y * b[k2].y;
+                const float y = 0.5f * b[k2].x;
+                const bool neg = (kind == 2 && (j & 1)) || (kind == 3 && !(j & 1));
+                emit(q, k2, j, kind == 0 ? x0 : neg ? -y : y);
+            }
+        }
+    }
+}
+
+// Values kept in registers between phase 2 and the staging, one per output.
+template <int N1, int N2>
+struct FftOut {
+    float v[((N1 + N2 - 1) / N2) * N2];
+};
+
+// The thread's outputs into a staged row (stride n + 1 floats).
+template <int N1, int N2>
+__device__ inline void fft_stage(float* stg, int r, int t, int kind, const FftOut<N1, N2>& o) {
+    constexpr int n = N1 * N2, NQ = (N1 + N2 - 1) / N2;
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+        if (t + N2 * q < N1) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; k2++)
+                stg[r * (n + 1) + fft_out_index<N1, N2>(kind, t, q, k2)] = o.v[q * N2 + k2];
+        }
+}
+
+// Store of rh staged rows of n: transposed (element k of row rr to
+// out[(k ld + r0 + rr) os], consecutive lanes on consecutive rows) or in
+// row order (out[((r0 + rr) n + k) os], consecutive lanes on consecutive k).
+template <int n, int R, bool ROWS>
+__device__ inline void fft_store(const float* stg, EP_GLOBAL float* out, int ld, int os, int r0, int rh) {
+    const int tid = threadIdx.x;
+    auto put = [&](int rr, int k) {
+        const size_t o = ROWS ? (size_t)(r0 + rr) * n + k : (size_t)k * ld + r0 + rr;
+        out[o * os] = stg[rr * (n + 1) + k];
+    };
+    if (rh == R) {
+#pragma unroll
+        for (int u = 0; u < R * n / 256; u++) {
+            const int i = tid + 256 * u;
+            if (ROWS) put(i / n, i % n);
+            else put(i % R, i / R);
+        }
+    } else {
+        const int lrh = 31 - __clz(rh);
+        for (int i = tid; i < rh * n; i += 256) {
+            if (ROWS) put(i / n, i % n);
+            else put(i & (rh - 1), i >> lrh);
+        }
+    }
+}
+
+// A row pass: rows r0 .. r0 + rh of the job's plane, one transform kind,
+// stored transposed (epi 0) or in row order (epi 2).
+template <int N1, int N2>
+__device__ inline void fft_rows(const FftJob& J, int r0, f32x2* ex) {
     constexpr int n = N1 * N2;
     constexpr int R = 256 / N2;
-    constexpr int PAD = N2 + 1;
-    constexpr int NQ = (N1 + N2 - 1) / N2;  // second-phase DFTs per thread (some idle when N1 < N2)
-    constexpr int NO = NQ * N2;              // outputs per thread of the second phase
-    static_assert(R * N1 * PAD <= kFftLds && R * (n + 1) <= 2 * kFftLds, "LDS");
+    static_assert(R * N1 * (N2 + 1) <= kFftLds && R * (n + 1) <= 2 * kFftLds, "LDS");
     const int tid = threadIdx.x;
     const int t = tid % N2, r = tid / N2;
     const int rh = min(R, J.rows - r0);  // rows of this block (a power of two)
@@ -115,134 +246,101 @@ __device__ inline void fft_rows(const FftJob& J, const DesignDev* dd, int r0, f3
     const EP_GLOBAL float* x = (const EP_GLOBAL float*)J.in + (size_t)(r0 + r) * n;
     const EP_GLOBAL f32x2* tw = (const EP_GLOBAL f32x2*)J.tw;
     const EP_GLOBAL f32x2* dtw = (const EP_GLOBAL f32x2*)J.dtw;
-
-    // phase 1: points j = t + N2 m, scaled by e^{-i pi j/2n} for the C3
-    // kinds; N1-point DFT; times e^{-2 pi i t k1/n}.  The loads are the
-    // same for every kind (two in-row elements and the factor; the kind only
-    // picks indices and selects): the address of a load never depends on a
-    // branch over the kind.
     if (live) {
         f32x2 a[N1];
-#pragma unroll
-        for (int m = 0; m < N1; m++) {
-            const int j = t + N2 * m;
-            // DCT-II: v_j = x_{2j} (j < n/2) or x_{2(n-1-j)+1}; C3 kinds:
-            // Z_j and Z_{n-j} (Z_n = 0), indices wrapped into the row where the
-            // element is zeroed below
-            const int i0 = j < n / 2 ? 2 * j : 2 * (n - 1 - j) + 1;
-            const int ia = kind == 0 ? i0 : kind == 1 ? j : kind == 2 ? n - 1 - j : (n - j) & (n - 1);
-            const int ib = kind == 1 ? (n - j) & (n - 1) : kind == 2 ? (j - 1) & (n - 1) : j;
-            const float xa = x[ia], xb = x[ib];
-            const f32x2 c = dtw[j];
-            const float za = (kind == 3 && j == 0) ? 0.f : xa;
-            const float zb = j == 0 ? 0.f : xb;
-            const f32x2 w = cmul(c, (f32x2){za, zb});
-            a[m] = kind == 0 ? (f32x2){xa, 0.f} : w;
-        }
-        dft_reg<N1>(a);
-#pragma unroll
-        for (int k1 = 0; k1 < N1; k1++) ex[(r * N1 + k1) * PAD + t] = cmul(a[k1], tw[t * k1]);
+        fft_input<N1, N2>(kind, t, dtw, [&](int i) { return x[i]; }, a);
+        fft_phase1<N1, N2>(a, tw, ex, r, t);
     }
     __syncthreads();
-    // phase 2: this thread's k1 = t + N2 q (< N1); outputs k = k1 + N1 k2
-    float val[NO];
-    int oj[NO];
-    if (live) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            const int k1 = t + N2 * q;
-            if (k1 < N1) {
-                f32x2 b[N2];
-#pragma unroll
-                for (int s2 = 0; s2 < N2; s2++) b[s2] = ex[(r * N1 + k1) * PAD + s2];
-                dft_reg<N2>(b);
-#pragma unroll
-                for (int k2 = 0; k2 < N2; k2++) {
-                    // DCT-II: X_k = Re(e^{-i pi k/2n} V_k) at k; C3 kinds:
-                    // Re(w_k) / 2 at j (signed for the sine kinds)
-                    const int u = q * N2 + k2, k = k1 + N1 * k2;
-                    const f32x2 c = dtw[k];
-                    const int j = k < n / 2 ? 2 * k : 2 * (n - 1 - k) + 1;
-                    const float x0 = c.x * b[k2].x - c.y * b[k2].y;
-                    const float y = 0.5f * b[k2].x;
-                    const bool neg = (kind == 2 && (j & 1)) || (kind == 3 && !(j & 1));
-                    oj[u] = kind == 0 ? k : j;
-                    val[u] = kind == 0 ? x0 : neg ? -y : y;
-                }
-            }
-        }
-    }
+    FftOut<N1, N2> o;
+    if (live)
+        fft_phase2<N1, N2>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v) { o.v[q * N2 + k2] = v; });
     __syncthreads();  // the exchange is read; its LDS now stages the outputs
     float* stg = (float*)ex;
-    if (live) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++)
-            if (t + N2 * q < N1) {
-#pragma unroll
-                for (int k2 = 0; k2 < N2; k2++) stg[r * (n + 1) + oj[q * N2 + k2]] = val[q * N2 + k2];
-            }
-    }
+    if (live) fft_stage<N1, N2>(stg, r, t, kind, o);
     __syncthreads();
-    // transposed store: consecutive lanes take consecutive rows of one output
-    // index; a full block stores R n / 256 rounds of 256 elements
-    const int ld = J.ld;
-    EP_GLOBAL float* out = (EP_GLOBAL float*)J.out;
-    if (J.epi == 1) {
-        // the Poisson solve (density.rs:113-153) at (ky = k, kx = r0 + rr)
-        const DesignDev& D = dd[J.design];
-        const EP_GLOBAL f32x2* sx = (const EP_GLOBAL f32x2*)D.spec_x;
-        const EP_GLOBAL f32x2* sy = (const EP_GLOBAL f32x2*)D.spec_y;
-        EP_GLOBAL float* out2 = (EP_GLOBAL float*)J.out2;
-        const float s0 = 4.0f / ((float)D.Bx * (float)D.By);
-        auto poisson_store = [&](int rr, int k) {
-            const int kx = r0 + rr;
-            const f32x2 fx = sx[kx], fy = sy[k];
-            float sc = s0;
-            if (kx == 0) sc *= 0.5f;
-            if (k == 0) sc *= 0.5f;
-            const float a = stg[rr * (n + 1) + k] * sc;
-            const float den = fx.y + fy.y;
-            float e0 = 0.f, e1 = 0.f;
-            if (!(den < 1e-30f)) {
-                const float phi = a / den;
-                e0 = phi * fx.x;
-                e1 = phi * fy.x;
-            }
-            const size_t o = (size_t)k * ld + kx;
-            out[o] = e0;
-            out2[o] = e1;
-        };
-        if (rh == R) {
-#pragma unroll
-            for (int u = 0; u < R * n / 256; u++) {
-                const int i = tid + 256 * u;
-                poisson_store(i % R, i / R);
-            }
-        } else {
-            const int lrh = 31 - __clz(rh);
-            for (int i = tid; i < rh * n; i += 256) poisson_store(i & (rh - 1), i >> lrh);
-        }
-        return;
-    }
-    const int os = J.ostride;
-    if (rh == R) {
-#pragma unroll
-        for (int u = 0; u < R * n / 256; u++) {
-            const int i = tid + 256 * u;
-            const int rr = i % R, k = i / R;
-            out[((size_t)k * ld + r0 + rr) * os] = stg[rr * (n + 1) + k];
-        }
-    } else {
-        const int lrh = 31 - __clz(rh);
-        for (int i = tid; i < rh * n; i += 256) {
-            const int rr = i & (rh - 1), k = i >> lrh;
-            out[((size_t)k * ld + r0 + rr) * os] = stg[rr * (n + 1) + k];
-        }
-    }
+    if (J.epi == 2) fft_store<n, R, true>(stg, (EP_GLOBAL float*)J.out, J.ld, J.ostride, r0, rh);
+    else fft_store<n, R, false>(stg, (EP_GLOBAL float*)J.out, J.ld, J.ostride, r0, rh);
 }
 
-__global__ __launch_bounds__(256) void k_fft_pass(const FftJob* jobs, const int32_t* fblk,
-                                                  const DesignDev* dd) {
+// The column solve (density.rs:113-153 between the two transforms): rows u
+// of the plane transformed along x, each taken through the DCT-II along y,
+// the Poisson solve phi = a s / (l_u + l_v) (s = 4 / (Bx By), halved at u = 0
+// and at v = 0; zero where the denominator vanishes), and both inverse
+// transforms along y: C3 of phi w_u (the Ex path) and the job's sine kind of
+// phi w_v (the Ey path), stored transposed into out (Ex) and out2 (Ey).  The
+// inverse along x then finishes each component in a row pass.
+template <int N1, int N2>
+__device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f32x2* ex, float* phi) {
+    constexpr int n = N1 * N2;
+    constexpr int R = 256 / N2;
+    static_assert(R * N1 * (N2 + 1) <= kSolveEx && R * (n + 1) <= kSolveRows &&
+                      R * (n + 1) <= 2 * kSolveEx, "LDS");
+    const int tid = threadIdx.x;
+    const int t = tid % N2, r = tid / N2;
+    const int rh = min(R, J.rows - r0);
+    const bool live = r < rh;
+    const int kx = r0 + r;
+    const EP_GLOBAL float* x = (const EP_GLOBAL float*)J.in + (size_t)kx * n;
+    const EP_GLOBAL f32x2* tw = (const EP_GLOBAL f32x2*)J.tw;
+    const EP_GLOBAL f32x2* dtw = (const EP_GLOBAL f32x2*)J.dtw;
+    const DesignDev& D = dd[J.design];
+    const EP_GLOBAL f32x2* sx = (const EP_GLOBAL f32x2*)D.spec_x;
+    const EP_GLOBAL f32x2* sy = (const EP_GLOBAL f32x2*)D.spec_y;
+    const float s0 = 4.0f / ((float)D.Bx * (float)D.By);
+    float* prow = phi + r * (n + 1);
+    // forward DCT-II along y, the Poisson solve written into the LDS rows
+    if (live) {
+        f32x2 a[N1];
+        fft_input<N1, N2>(0, t, dtw, [&](int i) { return x[i]; }, a);
+        fft_phase1<N1, N2>(a, tw, ex, r, t);
+    }
+    __syncthreads();
+    if (live) {
+        const f32x2 fx = sx[kx];
+        const float su = kx == 0 ? 0.5f * s0 : s0;
+        fft_phase2<N1, N2>(ex, r, t, 0, dtw, [&](int, int, int k, float v) {
+            const f32x2 fy = sy[k];
+            const float sc = k == 0 ? 0.5f * su : su;
+            const float den = fx.y + fy.y;
+            prow[k] = den < 1e-30f ? 0.f : v * sc / den;
+        });
+    }
+    __syncthreads();
+    // Ex path: C3 along y of phi w_u
+    FftOut<N1, N2> oa;
+    if (live) {
+        const float wu = sx[kx].x;
+        f32x2 a[N1];
+        fft_input<N1, N2>(1, t, dtw, [&](int i) { return prow[i] * wu; }, a);
+        fft_phase1<N1, N2>(a, tw, ex, r, t);
+    }
+    __syncthreads();
+    if (live)
+        fft_phase2<N1, N2>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v) { oa.v[q * N2 + k2] = v; });
+    __syncthreads();
+    // Ey path: the sine kind along y of phi w_v
+    const int kind = J.kind;
+    if (live) {
+        f32x2 a[N1];
+        fft_input<N1, N2>(kind, t, dtw, [&](int i) { return prow[i] * sy[i].x; }, a);
+        fft_phase1<N1, N2>(a, tw, ex, r, t);
+    }
+    __syncthreads();  // the Poisson rows are read: they now stage the Ex output
+    float* stA = phi;
+    if (live) fft_stage<N1, N2>(stA, r, t, 1, oa);
+    FftOut<N1, N2> ob;
+    if (live)
+        fft_phase2<N1, N2>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v) { ob.v[q * N2 + k2] = v; });
+    __syncthreads();  // the exchange is read; its LDS now stages the Ey output
+    float* stB = (float*)ex;
+    if (live) fft_stage<N1, N2>(stB, r, t, kind, ob);
+    __syncthreads();
+    fft_store<n, R, false>(stA, (EP_GLOBAL float*)J.out, J.ld, 1, r0, rh);
+    fft_store<n, R, false>(stB, (EP_GLOBAL float*)J.out2, J.ld, 1, r0, rh);
+}
+
+__global__ __launch_bounds__(256) void k_fft_pass(const FftJob* jobs, const int32_t* fblk) {
     __shared__ f32x2 ex[kFftLds];
     const int jb = ((const EP_GLOBAL int32_t*)fblk)[2 * blockIdx.x];
     if (jb < 0) return;
@@ -250,22 +348,49 @@ __global__ __launch_bounds__(256) void k_fft_pass(const FftJob* jobs, const int3
     const FftJob& J = jobs[jb];
     if (J.st->done) return;
     switch (J.n) {
-        case 16: fft_rows<4, 4>(J, dd, r0, ex); break;
-        case 32: fft_rows<8, 4>(J, dd, r0, ex); break;
-        case 64: fft_rows<8, 8>(J, dd, r0, ex); break;
-        case 128: fft_rows<16, 8>(J, dd, r0, ex); break;
-        case 256: fft_rows<16, 16>(J, dd, r0, ex); break;
-        case 512: fft_rows<16, 32>(J, dd, r0, ex); break;
+        case 16: fft_rows<4, 4>(J, r0, ex); break;
+        case 32: fft_rows<8, 4>(J, r0, ex); break;
+        case 64: fft_rows<8, 8>(J, r0, ex); break;
+        case 128: fft_rows<16, 8>(J, r0, ex); break;
+        case 256: fft_rows<16, 16>(J, r0, ex); break;
+        case 512: fft_rows<16, 32>(J, r0, ex); break;
+        default: break;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fft_solve(const FftJob* jobs, const int32_t* fblk,
+                                                   const DesignDev* dd) {
+    __shared__ f32x2 ex[kSolveEx];
+    __shared__ float phi[kSolveRows];
+    const int jb = ((const EP_GLOBAL int32_t*)fblk)[2 * blockIdx.x];
+    if (jb < 0) return;
+    const int r0 = ((const EP_GLOBAL int32_t*)fblk)[2 * blockIdx.x + 1];
+    const FftJob& J = jobs[jb];
+    if (J.st->done) return;
+    switch (J.n) {
+        case 16: fft_solve<4, 4>(J, dd, r0, ex, phi); break;
+        case 32: fft_solve<8, 4>(J, dd, r0, ex, phi); break;
+        case 64: fft_solve<8, 8>(J, dd, r0, ex, phi); break;
+        case 128: fft_solve<8, 16>(J, dd, r0, ex, phi); break;
+        case 256: fft_solve<16, 16>(J, dd, r0, ex, phi); break;
+        case 512: fft_solve<16, 32>(J, dd, r0, ex, phi); break;
         default: break;
     }
 }
 
 int fft_rows_per_block(int n) { return fft_block_rows(n); }
+int fft_solve_rows_per_block(int n) { return fft_solve_rows(n); }
 
 void launch_fft_pass(const FftJob* jobs, const int32_t* fblk, int blocks, const DesignDev* dd,
                      hipStream_t s) {
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(256), 0, s, jobs, fblk, dd);
+    hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(256), 0, s, jobs, fblk);
+}
+
+void launch_fft_solve(const FftJob* jobs, const int32_t* fblk, int blocks, const DesignDev* dd,
+                      hipStream_t s) {
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(k_fft_solve, dim3(blocks), dim3(256), 0, s, jobs, fblk, dd);
 }
 
 }  // namespace ep

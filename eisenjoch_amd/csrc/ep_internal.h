@@ -174,12 +174,14 @@ struct PlaneSum {
 };
 
 // One batch of row transforms of the FFT engine (FAST path, power-of-two
-// row lengths): rows x n input, row-major; element k of row r goes to
-// out[(k * ld + r) * ostride] (the plane transposed; ostride 2 writes one
-// component of an interleaved pair).  kind: 0 DCT-II, 1 DCT-III with the
-// k = 0 term halved (C3), 2 DST-III of the reference field (S3), 3 the IDXST
-// sine transform.  epi: 0 store, 1 the Poisson solve into out (Ex^) and out2
-// (Ey^), at (ky = k, kx = r).  A block transforms rpb rows.
+// row lengths): rows x n input, row-major.  kind: 0 DCT-II, 1 DCT-III with
+// the k = 0 term halved (C3), 2 DST-III of the reference field (S3), 3 the
+// IDXST sine transform.  Row passes: epi 0 stores element k of row r at
+// out[(k * ld + r) * ostride] (the plane transposed), epi 2 at
+// out[(r * n + k) * ostride] (row order; ostride 2 writes one component of
+// an interleaved pair).  Column-solve jobs (the second pass): DCT-II, the
+// Poisson solve, then C3 into out and `kind` into out2, both transposed.
+// A block transforms rpb rows.
 struct FftJob {
     const void* in;
     void* out;
@@ -250,5 +252,9 @@ int fft_rows_per_block(int n);
 // fblk: (job, first row) of each block of the grid (blocks pairs; -1: none).
 void launch_fft_pass(const FftJob* jobs_dev, const int32_t* fblk, int blocks, const DesignDev* dd,
                      hipStream_t s);
+// The column solve (the FFT engine's second pass); rows per block of it.
+void launch_fft_solve(const FftJob* jobs_dev, const int32_t* fblk, int blocks, const DesignDev* dd,
+                      hipStream_t s);
+int fft_solve_rows_per_block(int n);
 
 }  // namespace ep

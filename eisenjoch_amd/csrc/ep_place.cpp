@@ -160,24 +160,29 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         cst(i2b, true, ty, 2); dat(i2b, false, D.dev.uT[1]);
         const int eng = engine_for(c, b.prec, Bx, By);
         if (eng == kEngineFft) {
-            // the six products as row transforms of the same planes
+            // three passes over the same planes: rows along x (DCT-II, stored
+            // transposed), the column solve along y (DCT-II, Poisson, C3 for
+            // Ex and the sine kind for Ey, stored transposed), rows along x
+            // (the sine kind for Ex, C3 for Ey) into the field in row order
             const int sk = cfgs[i]->field_mode == EP_FIELD_REFERENCE ? 2 : 3;
             auto fj = [&](const void* in, void* out, void* out2, const Transform* t, int n, int rows,
-                          int kind, int epi, int ld, int os) {
+                          int kind, int epi, int ld, int os, int rpb) {
                 FftJob F;
                 memset(&F, 0, sizeof(F));
                 F.in = in; F.out = out; F.out2 = out2;
                 F.tw = t->fft_tw; F.dtw = t->dct_tw; F.st = D.dev.st;
-                F.n = n; F.rows = rows; F.rpb = std::min(rows, fft_rows_per_block(n));
+                F.n = n; F.rows = rows; F.rpb = std::min(rows, rpb);
                 F.kind = kind; F.epi = epi; F.ld = ld; F.ostride = os; F.design = i;
                 return F;
             };
-            fjobs[0].push_back(fj(D.dev.dens, D.dev.tT, nullptr, tx, Bx, By, 0, 0, By, 1));
-            fjobs[1].push_back(fj(D.dev.tT, D.dev.ehat[0], D.dev.ehat[1], ty, By, Bx, 0, 1, Bx, 1));
-            fjobs[2].push_back(fj(D.dev.ehat[0], D.dev.uT[0], nullptr, tx, Bx, By, sk, 0, By, 1));
-            fjobs[2].push_back(fj(D.dev.ehat[1], D.dev.uT[1], nullptr, tx, Bx, By, 1, 0, By, 1));
-            fjobs[3].push_back(fj(D.dev.uT[0], D.dev.field, nullptr, ty, By, Bx, 1, 0, Bx, 2));
-            fjobs[3].push_back(fj(D.dev.uT[1], (char*)D.dev.field + tsz, nullptr, ty, By, Bx, sk, 0, Bx, 2));
+            fjobs[0].push_back(fj(D.dev.dens, D.dev.tT, nullptr, tx, Bx, By, 0, 0, By, 1,
+                                  fft_rows_per_block(Bx)));
+            fjobs[1].push_back(fj(D.dev.tT, D.dev.uT[0], D.dev.uT[1], ty, By, Bx, sk, 0, Bx, 1,
+                                  fft_solve_rows_per_block(By)));
+            fjobs[3].push_back(fj(D.dev.uT[0], D.dev.field, nullptr, tx, Bx, By, sk, 2, Bx, 2,
+                                  fft_rows_per_block(Bx)));
+            fjobs[3].push_back(fj(D.dev.uT[1], (char*)D.dev.field + tsz, nullptr, tx, Bx, By, 1, 2, Bx, 2,
+                                  fft_rows_per_block(Bx)));
             continue;
         }
         if (eng == 4) {
@@ -461,7 +466,10 @@ static void enqueue_gemm(Ctx* c, Batch& b, int p) {
         if (b.njobs2[p][e] > 0)
             launch_gemm_pass(b.jobs2[p][e], b.bjob2[p][e], b.tiles2[p][e], b.d_dev, b.prec, e, p,
                              c->stream);
-    if (b.fft_blocks[p] > 0) launch_fft_pass(b.fjobs[p], b.fblk[p], b.fft_blocks[p], b.d_dev, c->stream);
+    if (b.fft_blocks[p] > 0) {
+        if (p == 1) launch_fft_solve(b.fjobs[p], b.fblk[p], b.fft_blocks[p], b.d_dev, c->stream);
+        else launch_fft_pass(b.fjobs[p], b.fblk[p], b.fft_blocks[p], b.d_dev, c->stream);
+    }
     if (p == 3) launch_plane_sum(b.sums, b.n_sums, b.sum_blocks, c->stream);  // field = sum of planes
 }
 

@@ -1,12 +1,14 @@
-"""The FFT pass kernel built for the CPU (no GPU needed).
+"""The FFT engine's kernels built for the CPU (no GPU needed).
 
-The source of k_fft_pass (eisenjoch_amd/csrc/ep_fft.hip) is compiled with
-the host compiler and AddressSanitizer and run with one std::thread per GPU
-thread and a std::barrier for each __syncthreads, on buffers of exactly the
-sizes the kernel may touch: every supported row length, row counts below
-and above one block, every transform kind, the Poisson epilogue and the
-interleaved store.  Any access outside a buffer aborts the run; the outputs
-must equal direct O(n^2) transforms to 2e-6 of their largest value.
+The source of k_fft_pass and k_fft_solve (eisenjoch_amd/csrc/ep_fft.hip) is
+compiled with the host compiler and AddressSanitizer and run with one
+std::thread per GPU thread and a std::barrier for each __syncthreads, on
+buffers of exactly the sizes the kernels may touch.  Each case runs the
+engine's three passes on a random density, with jobs and block tables laid
+out as the library lays them out, for square and non-square shapes (row
+counts below and above one block) and both field modes; any access outside
+a buffer aborts the run, and the field must equal the FP64 oracle's
+(oracle/oracle.c) to 2e-6 of its largest value.
 """
 import shutil
 import subprocess
@@ -18,6 +20,8 @@ ROOT = Path(__file__).resolve().parents[1]
 SRC = ROOT / "eisenjoch_amd" / "csrc" / "ep_fft.hip"
 MAIN = Path(__file__).resolve().parent / "emu" / "fft_emu_main.cpp"
 CXX = shutil.which("clang++") or "/opt/rocm/llvm/bin/clang++"
+CC = shutil.which("clang") or "/opt/rocm/llvm/bin/clang"
+SAN = ["-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer"]
 
 
 @pytest.fixture(scope="module")
@@ -26,17 +30,23 @@ def emu(tmp_path_factory):
     s = SRC.read_text()
     body = s[s.index("typedef float f32x2"):s.index("int fft_rows_per_block(int n)")]
     (d / "fft_kernel_extract.inc").write_text(body)
-    exe = d / "fft_emu"
-    r = subprocess.run([CXX, "-std=c++20", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer",
-                        f"-I{d}", str(MAIN), "-o", str(exe), "-lpthread"],
-                       capture_output=True, text=True)
+    orc = d / "oracle.o"
+    r = subprocess.run([CC, *SAN, "-c", str(ROOT / "oracle" / "oracle.c"), f"-I{ROOT / 'oracle'}",
+                        "-o", str(orc)], capture_output=True, text=True)
     if r.returncode != 0:
-        pytest.fail(f"CPU build of the FFT kernel failed:\n{r.stderr[-3000:]}")
+        pytest.fail(f"oracle build failed:\n{r.stderr[-3000:]}")
+    exe = d / "fft_emu"
+    r = subprocess.run([CXX, "-std=c++20", *SAN, f"-I{d}", str(MAIN), str(orc), "-o", str(exe),
+                        "-lpthread", "-lm"], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.fail(f"CPU build of the FFT kernels failed:\n{r.stderr[-3000:]}")
     return exe
 
 
-@pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512])
-def test_fft_kernel_on_cpu(emu, n):
-    r = subprocess.run([str(emu), str(n)], capture_output=True, text=True, timeout=600)
+@pytest.mark.parametrize("Bx,By,mode", [(16, 16, 1), (16, 32, 0), (64, 64, 1), (128, 64, 0),
+                                        (32, 256, 1), (256, 256, 0), (512, 16, 1), (512, 512, 1)])
+def test_fft_engine_on_cpu(emu, Bx, By, mode):
+    r = subprocess.run([str(emu), str(Bx), str(By), str(mode)], capture_output=True, text=True,
+                       timeout=900)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-3000:]
     assert "emulation ok" in r.stdout
