@@ -340,6 +340,90 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
     fft_store<n, R, false>(stB, (EP_GLOBAL float*)J.out2, J.ld, 1, r0, rh);
 }
 
+// The last pass for both components at once: rows r0 .. r0 + rh of plane in
+// (the job's sine kind along x, Ex) and of plane in2 (C3 along x, Ey), stored
+// in row order as (Ex, Ey) pairs into out.  The second plane's points are
+// loaded before the first plane's exchange is read, so their latency
+// overlaps it.
+template <int N1, int N2>
+__device__ inline void fft_pair(const FftJob& J, int r0, f32x2* ex) {
+    constexpr int n = N1 * N2;
+    constexpr int R = 256 / N2;
+    static_assert(R * N1 * (N2 + 1) <= kFftLds && 2 * R * (n + 1) <= 2 * kFftLds, "LDS");
+    const int tid = threadIdx.x;
+    const int t = tid % N2, r = tid / N2;
+    const int rh = min(R, J.rows - r0);
+    const bool live = r < rh;
+    const int kind = J.kind;
+    const EP_GLOBAL float* xa = (const EP_GLOBAL float*)J.in + (size_t)(r0 + r) * n;
+    const EP_GLOBAL float* xb = (const EP_GLOBAL float*)J.in2 + (size_t)(r0 + r) * n;
+    const EP_GLOBAL f32x2* tw = (const EP_GLOBAL f32x2*)J.tw;
+    const EP_GLOBAL f32x2* dtw = (const EP_GLOBAL f32x2*)J.dtw;
+    f32x2 b[N1];
+    if (live) {
+        f32x2 a[N1];
+        fft_input<N1, N2>(kind, t, dtw, [&](int i) { return xa[i]; }, a);
+        fft_input<N1, N2>(1, t, dtw, [&](int i) { return xb[i]; }, b);
+        fft_phase1<N1, N2>(a, tw, ex, r, t);
+    }
+    __syncthreads();
+    FftOut<N1, N2> oa, ob;
+    if (live)
+        fft_phase2<N1, N2>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v) { oa.v[q * N2 + k2] = v; });
+    __syncthreads();
+    if (live) fft_phase1<N1, N2>(b, tw, ex, r, t);
+    __syncthreads();
+    if (live)
+        fft_phase2<N1, N2>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v) { ob.v[q * N2 + k2] = v; });
+    __syncthreads();  // the exchange is read; its LDS now stages (Ex, Ey) pairs
+    f32x2* stg = ex;
+    if (live) {
+        constexpr int NQ = (N1 + N2 - 1) / N2;
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            if (t + N2 * q < N1) {
+#pragma unroll
+                for (int k2 = 0; k2 < N2; k2++) {
+                    stg[r * (n + 1) + fft_out_index<N1, N2>(kind, t, q, k2)].x = oa.v[q * N2 + k2];
+                    stg[r * (n + 1) + fft_out_index<N1, N2>(1, t, q, k2)].y = ob.v[q * N2 + k2];
+                }
+            }
+    }
+    __syncthreads();
+    EP_GLOBAL f32x2* out = (EP_GLOBAL f32x2*)J.out;
+    if (rh == R) {
+#pragma unroll
+        for (int u = 0; u < R * n / 256; u++) {
+            const int i = tid + 256 * u;
+            const int rr = i / n, k = i % n;
+            out[(size_t)(r0 + rr) * n + k] = stg[rr * (n + 1) + k];
+        }
+    } else {
+        for (int i = tid; i < rh * n; i += 256) {
+            const int rr = i / n, k = i % n;
+            out[(size_t)(r0 + rr) * n + k] = stg[rr * (n + 1) + k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fft_pair(const FftJob* jobs, const int32_t* fblk) {
+    __shared__ f32x2 ex[kFftLds];
+    const int jb = ((const EP_GLOBAL int32_t*)fblk)[2 * blockIdx.x];
+    if (jb < 0) return;
+    const int r0 = ((const EP_GLOBAL int32_t*)fblk)[2 * blockIdx.x + 1];
+    const FftJob& J = jobs[jb];
+    if (J.st->done) return;
+    switch (J.n) {
+        case 16: fft_pair<4, 4>(J, r0, ex); break;
+        case 32: fft_pair<8, 4>(J, r0, ex); break;
+        case 64: fft_pair<8, 8>(J, r0, ex); break;
+        case 128: fft_pair<16, 8>(J, r0, ex); break;
+        case 256: fft_pair<16, 16>(J, r0, ex); break;
+        case 512: fft_pair<16, 32>(J, r0, ex); break;
+        default: break;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fft_pass(const FftJob* jobs, const int32_t* fblk) {
     __shared__ f32x2 ex[kFftLds];
     const int jb = ((const EP_GLOBAL int32_t*)fblk)[2 * blockIdx.x];
@@ -385,6 +469,11 @@ void launch_fft_pass(const FftJob* jobs, const int32_t* fblk, int blocks, const 
                      hipStream_t s) {
     if (blocks <= 0) return;
     hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(256), 0, s, jobs, fblk);
+}
+
+void launch_fft_pair(const FftJob* jobs, const int32_t* fblk, int blocks, hipStream_t s) {
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(k_fft_pair, dim3(blocks), dim3(256), 0, s, jobs, fblk);
 }
 
 void launch_fft_solve(const FftJob* jobs, const int32_t* fblk, int blocks, const DesignDev* dd,

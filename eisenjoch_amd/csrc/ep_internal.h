@@ -181,9 +181,12 @@ struct PlaneSum {
 // out[(r * n + k) * ostride] (row order; ostride 2 writes one component of
 // an interleaved pair).  Column-solve jobs (the second pass): DCT-II, the
 // Poisson solve, then C3 into out and `kind` into out2, both transposed.
-// A block transforms rpb rows.
+// Pair jobs (the last pass): rows of in through `kind` and rows of in2
+// through C3, stored in row order as pairs into out.  A block transforms
+// rpb rows.
 struct FftJob {
     const void* in;
+    const void* in2;     // the last pass: the second component's plane
     void* out;
     void* out2;
     const void* tw;      // float2[n]: exp(-2 pi i t / n)
@@ -256,5 +259,7 @@ void launch_fft_pass(const FftJob* jobs_dev, const int32_t* fblk, int blocks, co
 void launch_fft_solve(const FftJob* jobs_dev, const int32_t* fblk, int blocks, const DesignDev* dd,
                       hipStream_t s);
 int fft_solve_rows_per_block(int n);
+// The last pass, both components per block (pair jobs).
+void launch_fft_pair(const FftJob* jobs_dev, const int32_t* fblk, int blocks, hipStream_t s);
 
 }  // namespace ep

@@ -179,10 +179,10 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
                                   fft_rows_per_block(Bx)));
             fjobs[1].push_back(fj(D.dev.tT, D.dev.uT[0], D.dev.uT[1], ty, By, Bx, sk, 0, Bx, 1,
                                   fft_solve_rows_per_block(By)));
-            fjobs[3].push_back(fj(D.dev.uT[0], D.dev.field, nullptr, tx, Bx, By, sk, 2, Bx, 2,
-                                  fft_rows_per_block(Bx)));
-            fjobs[3].push_back(fj(D.dev.uT[1], (char*)D.dev.field + tsz, nullptr, tx, Bx, By, 1, 2, Bx, 2,
-                                  fft_rows_per_block(Bx)));
+            FftJob last = fj(D.dev.uT[0], D.dev.field, nullptr, tx, Bx, By, sk, 2, Bx, 2,
+                             fft_rows_per_block(Bx));
+            last.in2 = D.dev.uT[1];
+            fjobs[3].push_back(last);
             continue;
         }
         if (eng == 4) {
@@ -468,6 +468,7 @@ static void enqueue_gemm(Ctx* c, Batch& b, int p) {
                              c->stream);
     if (b.fft_blocks[p] > 0) {
         if (p == 1) launch_fft_solve(b.fjobs[p], b.fblk[p], b.fft_blocks[p], b.d_dev, c->stream);
+        else if (p == 3) launch_fft_pair(b.fjobs[p], b.fblk[p], b.fft_blocks[p], c->stream);
         else launch_fft_pass(b.fjobs[p], b.fblk[p], b.fft_blocks[p], b.d_dev, c->stream);
     }
     if (p == 3) launch_plane_sum(b.sums, b.n_sums, b.sum_blocks, c->stream);  // field = sum of planes

@@ -31,7 +31,7 @@ static std::barrier<>* g_bar;
 struct DevState { int done; };
 struct DesignDev { int Bx, By; const void* spec_x; const void* spec_y; };
 struct FftJob {
-    const void* in; void* out; void* out2; const void* tw; const void* dtw; const DevState* st;
+    const void* in; const void* in2; void* out; void* out2; const void* tw; const void* dtw; const DevState* st;
     int32_t n, rows, rpb, kind, epi, ld, ostride, design;
 };
 #include "fft_kernel_extract.inc"
@@ -87,16 +87,16 @@ int main(int argc, char** argv) {
     const int sk = mode == 0 ? 2 : 3;
     auto job = [&](const void* in, void* out, void* out2, const std::vector<float>& tw,
                    const std::vector<float>& dtw, int n, int rows, int kind, int epi, int ld, int os, int rpb) {
-        return FftJob{in, out, out2, tw.data(), dtw.data(), &st, n, rows, std::min(rows, rpb), kind, epi, ld, os, 0};
+        return FftJob{in, nullptr, out, out2, tw.data(), dtw.data(), &st, n, rows, std::min(rows, rpb), kind, epi, ld, os, 0};
     };
     std::vector<FftJob> p1{job(dens.data(), tT.data(), nullptr, twx, dtwx, Bx, By, 0, 0, By, 1, fft_block_rows(Bx))};
     std::vector<FftJob> p2{job(tT.data(), u0.data(), u1.data(), twy, dtwy, By, Bx, sk, 0, Bx, 1, fft_solve_rows(By))};
-    std::vector<FftJob> p3{job(u0.data(), field.data(), nullptr, twx, dtwx, Bx, By, sk, 2, Bx, 2, fft_block_rows(Bx)),
-                           job(u1.data(), field.data() + 1, nullptr, twx, dtwx, Bx, By, 1, 2, Bx, 2, fft_block_rows(Bx))};
+    std::vector<FftJob> p3{job(u0.data(), field.data(), nullptr, twx, dtwx, Bx, By, sk, 2, Bx, 2, fft_block_rows(Bx))};
+    p3[0].in2 = u1.data();
     auto f1 = blocks(p1), f2 = blocks(p2), f3 = blocks(p3);
     run(f1, [&] { k_fft_pass(p1.data(), f1.data()); });
     run(f2, [&] { k_fft_solve(p2.data(), f2.data(), &D); });
-    run(f3, [&] { k_fft_pass(p3.data(), f3.data()); });
+    run(f3, [&] { k_fft_pair(p3.data(), f3.data()); });
     std::vector<double> ex(NB), ey(NB);
     orc_field_mode(densd.data(), Bx, By, 1.0, mode, ex.data(), ey.data());
     double err = 0, mx = 0;
