@@ -8,8 +8,9 @@ whole batch from its initial positions until every design reaches overflow
 <= 0.10 (from iteration 20 on; checked every iteration by default, every 5
 with --stop reference as the reference's loop does), all designs advancing
 together in one hipGraph-replayed launch
-sequence per iteration; FP32 bulk arithmetic, BF16 hi+lo MFMA transforms,
-FP64 reductions.  The spectral field is the index-shifted (IDXST) one: the
+sequence per iteration; FP32 bulk arithmetic, transforms on the engine the
+autotuner picks per bin shape (FP32 FFTs on these power-of-two shapes; BF16
+hi+lo MFMA products are the other candidates), FP64 reductions.  The spectral field is the index-shifted (IDXST) one: the
 reference's own field never reaches overflow 0.10 on these designs.
 
 value = designs placed per second over the timed steps (inputs resident in
@@ -82,7 +83,7 @@ def place_cfg(args):
 
 def single(args):
     """One design: the plain FP64 GPU placer against the FAST one
-    (FP32 bulk, BF16 hi+lo MFMA transforms, FP64 reductions), same seeded
+    (FP32 bulk, autotuned transform engine, FP64 reductions), same seeded
     netlist and initial positions; for config 3 also the FAST field's error
     against the FP64 field on the same density."""
     import eisenjoch_amd as ep
@@ -237,7 +238,8 @@ def serve(args):
         "config": {"workload": "serving: Poisson arrivals, log-uniform 1k-500k cells, 128 slots, 200 GB",
                    "pool": len(pool), "seed": 1,
                    "stop_check": "every iteration" if args.stop == "every" else "every 5 iterations"},
-        "dtype": "f32+bf16x3-mfma+f64-reductions", "data": "synthetic (seeded generator)",
+        "dtype": "f32 (FFT transforms up to 512 bins, BF16 hi+lo MFMA above)+f64-reductions",
+        "data": "synthetic (seeded generator)",
         "serving_stats": {**st, "batch_builds_per_s": round(st["batch_builds"] / wall, 2),
                           "graph_captures_per_s": round(st["graph_captures"] / wall, 2),
                           "graph_updates_per_s": round(st["graph_updates"] / wall, 2)},
@@ -262,7 +264,7 @@ def algorithmic_work(designs):
     """Per-iteration algorithmic bytes/flops of each kernel, summed over the
     batch (FP32 bulk, 4-byte indices)."""
     wa = scat = gath = 0.0
-    dct = 0.0
+    dct = fft = 0.0
     for d in designs:
         N, F, E, P = d.n_mov, d.n_fixed, d.n_nets, d.n_pins
         B2 = d.Bx * d.By
@@ -272,7 +274,10 @@ def algorithmic_work(designs):
         # six products of 2 M N K flops (three with K = Bx, three with K = By),
         # three BF16 MFMA products per FP32 one: 18 Bx By (Bx + By) = 36 B^3
         dct += 3 * 2 * d.Bx * d.By * (3 * d.Bx + 3 * d.By)
-    return {"wa": wa, "scatter": scat, "gather+update": gath, "dct_flops": dct}
+        # the FFT engine's three passes: density -> transposed rows (4 + 4),
+        # column solve in -> two planes out (4 + 8), two planes -> the field (8 + 8)
+        fft += 36 * B2
+    return {"wa": wa, "scatter": scat, "gather+update": gath, "dct_flops": dct, "fft_bytes": fft}
 
 
 # Kernel groups of one batched iteration: the profiler phases they span, the
@@ -292,6 +297,27 @@ KERNEL_GROUPS = {
         "phases": ["dct_f1", "dct_f2", "dct_i1", "dct_i2"], "bound": "mfma", "work": "dct_flops",
         "kernels": {"ep::k_gemm_mfma<": 4}},
 }
+# The transforms on the FFT engine (every shape of the batch on it): three
+# passes, priced by the bytes of the planes they read and write.
+FFT_GROUP = ("transforms (3 FFT passes, FP32)", {
+    "phases": ["dct_f1", "dct_f2", "dct_i2"], "bound": "hbm", "work": "fft_bytes",
+    "kernels": {"ep::k_fft_pass(": 1, "ep::k_fft_solve(": 1, "ep::k_fft_pair(": 1}})
+
+
+def kernel_groups(engines):
+    """The kernel groups of the batch, with the transform group of the engine
+    the autotuner chose (FFT when every shape took it)."""
+    g = dict(KERNEL_GROUPS)
+    if engines and all(v == "fft" for v in engines.values()):
+        del g["dct (4 transform passes, BF16 hi+lo MFMA)"]
+        g[FFT_GROUP[0]] = FFT_GROUP[1]
+    return g
+
+
+def dtype_of(engines):
+    if engines and all(v == "fft" for v in engines.values()):
+        return "f32 (FFT transforms)+f64-reductions"
+    return "f32+bf16x3-mfma+f64-reductions"
 
 
 def group_traffic(path, group):
@@ -435,7 +461,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+bf16x3-mfma+f64-reductions",
+        "dtype": dtype_of(tune),
         "data": "synthetic (seeded generator)",
         "config": {
             "workload": "batched: 64 designs, log-uniform 2k-100k cells, nets 1.1x, side sqrt(N/0.7), "
@@ -482,7 +508,7 @@ def main():
     # roofline of the dominant kernel group, HIP events on the context stream
     prof = dev.profile(ids, cfg, warm=8, reps=5)
     work = algorithmic_work(designs)
-    groups = {name: (sum(prof[ph] for ph in g["phases"]), g) for name, g in KERNEL_GROUPS.items()}
+    groups = {name: (sum(prof[ph] for ph in g["phases"]), g) for name, g in kernel_groups(tune).items()}
     dom = max(groups, key=lambda k: groups[k][0])
     ms, g = groups[dom]
     traffic = group_traffic(args.traffic_file, g)

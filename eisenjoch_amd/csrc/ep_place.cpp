@@ -54,13 +54,15 @@ int validate_config(Ctx* c, const ep_config* cfg, int log_cap) {
 }
 
 // Transform engine of a design shape: the autotuned (or pinned) choice when
-// there is one, else MFMA with LDS staging in K stages of 32 for BF16-eligible
-// shapes (bins multiples of 64 on the FAST path) and the FMA kernel otherwise.
-// The FP64 path always takes the FMA kernel.
+// there is one, else the FFT engine for bin counts that are powers of two up
+// to 512, MFMA with LDS staging in K stages of 32 for other shapes whose bins
+// are multiples of 64, and the FMA kernel otherwise.  The FP64 path always
+// takes the FMA kernel.
 int engine_for(Ctx* c, int prec, int Bx, int By) {
     if (prec != EP_PREC_FAST || getenv("EP_FORCE_FMA") != nullptr) return 1;
     auto it = c->tune.find({Bx, By});
     if (it != c->tune.end()) return it->second;
+    if (fft_len_ok(Bx) && fft_len_ok(By)) return kEngineFft;
     return (Bx % 64 == 0 && By % 64 == 0) ? 0 : 1;
 }
 
@@ -830,11 +832,12 @@ int ep_iter_log(ep_ctx* h, uint32_t id, ep_iter_rec* recs, uint32_t cap, uint32_
 // ---------------------------------------------------------- autotuner ----
 
 // Chooses the transform engine of every distinct bin shape among `ids` by
-// timing the four transform passes of the whole batch `ids` (the launches
-// the placement loop makes, so each candidate is measured under the
-// occupancy the other shapes' tiles impose), one shape at a time from the
-// largest, each candidate with the other shapes at their current choice;
-// the fastest is kept in the context's table.  With a cache path, shapes
+// timing the transform passes of the whole batch `ids` (the launches the
+// placement loop makes, so each candidate is measured under the occupancy
+// the other shapes' tiles impose): first every new shape on one engine (MFMA,
+// FFT, MFMA K64, where eligible), the fastest kept as the start; then one
+// shape at a time from the largest, each candidate with the other shapes at
+// their current choice; the fastest is kept in the context's table.  With a cache path, shapes
 // already in the file are taken from it and new results are appended.
 // Engines differ only in FP32 rounding order, and the table is per shape,
 // so a design's results do not depend on the batch it runs in.
@@ -920,6 +923,28 @@ extern "C" int ep_autotune(ep_ctx* h, const uint32_t* ids, uint32_t n, uint32_t 
     });
     for (auto& sh : todo)
         if (!c->tune.count(sh)) c->tune[sh] = engine_for(c, EP_PREC_FAST, sh.first, sh.second);
+    // uniform starting points first: every new shape on one engine where it
+    // is eligible (a pass whose shapes take two engines makes two launches,
+    // which a shape-by-shape search starting from one engine never leaves)
+    if (!todo.empty()) {
+        std::map<std::pair<int, int>, int> start = c->tune;
+        float best = 1e30f;
+        for (int e : {0, kEngineFft, 3}) {
+            std::map<std::pair<int, int>, int> trial = start;
+            bool any = false;
+            for (auto& sh : todo) {
+                const std::vector<int> ok = engines_for_shape(sh.first, sh.second);
+                if (std::find(ok.begin(), ok.end(), e) != ok.end()) { trial[sh] = e; any = true; }
+            }
+            if (!any) continue;
+            c->tune = trial;
+            float ms = 0.f;
+            int rc = time_transforms(c, ds, reps, &ms);
+            if (rc) return rc;
+            if (ms < best) { best = ms; start = trial; }
+        }
+        c->tune = start;
+    }
     std::vector<std::pair<std::pair<int, int>, int>> fresh;
     for (auto& sh : todo) {
         float best = 1e30f;
