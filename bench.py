@@ -65,6 +65,9 @@ def parse():
                    help="overflow check every iteration (default: the first iteration that "
                         "reaches overflow <= 0.10 ends a design) or every 5 iterations as the "
                         "reference's loop, which also legalizes there, does")
+    p.add_argument("--engine", default="auto", choices=["auto", "mfma", "fft"],
+                   help="transform engine of every bin shape: autotuned (default), or pinned to "
+                        "MFMA (engine 0) or FFT (engine 5) to profile one engine")
     p.add_argument("--traffic-file", default=str(ROOT / "profiles" / "pmc_traffic_batched64.json"),
                    help="per-dispatch FETCH_SIZE/WRITE_SIZE of the batched iteration's kernels, "
                         "from rocprofv3 --pmc runs of --profile-only (reported as roofline.traffic)")
@@ -422,7 +425,12 @@ def main():
     ids = [dev.upload(d, "fast") for d in designs]
     # transform engine per bin shape, chosen by timing each candidate (setup,
     # like the graph capture of the first placement: outside the timed steps)
-    tune = dev.autotune(ids, reps=5)
+    if args.engine == "auto":
+        tune = dev.autotune(ids, reps=5)
+    else:
+        for bx, by in sorted({(d.Bx, d.By) for d in designs}):
+            dev.tune_set(bx, by, 0 if args.engine == "mfma" else 5)
+        tune = dev.tune_table()
 
     if args.profile_only:
         prof = dev.profile(ids, cfg, warm=8, reps=5)
