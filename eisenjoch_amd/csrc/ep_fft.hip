@@ -91,7 +91,7 @@ __device__ inline void dft_reg(f32x2 (&a)[N]) {
 }
 
 // Split of each supported length n = N1 N2 (N1 points per thread in the
-// first phase, N2 threads per row; a block of 256 threads takes 256 / N2
+// first phase, N2 threads per row; a block of kFftThreads threads takes kFftThreads / N2
 // rows).  Row passes: 16 = 4 x 4, 32 = 8 x 4, 64 = 8 x 8, 128 = 16 x 8,
 // 256 = 16 x 16, 512 = 16 x 32.  The column solve keeps more in LDS per row
 // and takes 128 as 8 x 16.
@@ -101,15 +101,19 @@ __host__ __device__ constexpr int fft_n2(int n) {
 __host__ __device__ constexpr int fft_n2_solve(int n) {
     return n >= 512 ? 32 : n >= 128 ? 16 : n >= 64 ? 8 : 4;
 }
-__host__ __device__ constexpr int fft_block_rows(int n) { return 256 / fft_n2(n); }
-__host__ __device__ constexpr int fft_solve_rows(int n) { return 256 / fft_n2_solve(n); }
+// Threads per block of the FFT kernels.
+constexpr int kFftThreads = 256;
+__host__ __device__ constexpr int fft_block_rows(int n) { return kFftThreads / fft_n2(n); }
+__host__ __device__ constexpr int fft_solve_rows(int n) { return kFftThreads / fft_n2_solve(n); }
 
-// LDS of the row pass (largest case: n = 128, 32 rows x 16 x (8 + 1) points)
-constexpr int kFftLds = 32 * 16 * 9;  // f32x2 elements
-// LDS of the column solve: the exchange (largest: n = 256, 16 x 16 x 17
-// points) and the Poisson-solved rows (largest: 16 rows x 257)
-constexpr int kSolveEx = 16 * 16 * 17;   // f32x2 elements
-constexpr int kSolveRows = 16 * 257;     // floats
+// LDS of the row pass (largest case: n = 128, kFftThreads / 8 rows x 16 x
+// (8 + 1) points)
+constexpr int kFftLds = (kFftThreads / 8) * 16 * 9;  // f32x2 elements
+// LDS of the column solve: the exchange (largest: n = 256, kFftThreads / 16
+// rows x 16 x 17 points) and the Poisson-solved rows (largest: n = 256,
+// kFftThreads / 16 rows x 257)
+constexpr int kSolveEx = (kFftThreads / 16) * 16 * 17;  // f32x2 elements
+constexpr int kSolveRows = (kFftThreads / 16) * 257;    // floats
 
 // The complex input points j = t + N2 m (m < N1) of thread t's share of one
 // row, for the transform kind: DCT-II takes v_j = x_{2j} (j < n/2) or
@@ -217,14 +221,14 @@ __device__ inline void fft_store(const float* stg, EP_GLOBAL float* out, int ld,
     };
     if (rh == R) {
 #pragma unroll
-        for (int u = 0; u < R * n / 256; u++) {
-            const int i = tid + 256 * u;
+        for (int u = 0; u < R * n / kFftThreads; u++) {
+            const int i = tid + kFftThreads * u;
             if (ROWS) put(i / n, i % n);
             else put(i % R, i / R);
         }
     } else {
         const int lrh = 31 - __clz(rh);
-        for (int i = tid; i < rh * n; i += 256) {
+        for (int i = tid; i < rh * n; i += kFftThreads) {
             if (ROWS) put(i / n, i % n);
             else put(i & (rh - 1), i >> lrh);
         }
@@ -236,7 +240,7 @@ __device__ inline void fft_store(const float* stg, EP_GLOBAL float* out, int ld,
 template <int N1, int N2>
 __device__ inline void fft_rows(const FftJob& J, int r0, f32x2* ex) {
     constexpr int n = N1 * N2;
-    constexpr int R = 256 / N2;
+    constexpr int R = kFftThreads / N2;
     static_assert(R * N1 * (N2 + 1) <= kFftLds && R * (n + 1) <= 2 * kFftLds, "LDS");
     const int tid = threadIdx.x;
     const int t = tid % N2, r = tid / N2;
@@ -273,7 +277,7 @@ __device__ inline void fft_rows(const FftJob& J, int r0, f32x2* ex) {
 template <int N1, int N2>
 __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f32x2* ex, float* phi) {
     constexpr int n = N1 * N2;
-    constexpr int R = 256 / N2;
+    constexpr int R = kFftThreads / N2;
     static_assert(R * N1 * (N2 + 1) <= kSolveEx && R * (n + 1) <= kSolveRows &&
                       R * (n + 1) <= 2 * kSolveEx, "LDS");
     const int tid = threadIdx.x;
@@ -348,7 +352,7 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
 template <int N1, int N2>
 __device__ inline void fft_pair(const FftJob& J, int r0, f32x2* ex) {
     constexpr int n = N1 * N2;
-    constexpr int R = 256 / N2;
+    constexpr int R = kFftThreads / N2;
     static_assert(R * N1 * (N2 + 1) <= kFftLds && 2 * R * (n + 1) <= 2 * kFftLds, "LDS");
     const int tid = threadIdx.x;
     const int t = tid % N2, r = tid / N2;
@@ -393,20 +397,20 @@ __device__ inline void fft_pair(const FftJob& J, int r0, f32x2* ex) {
     EP_GLOBAL f32x2* out = (EP_GLOBAL f32x2*)J.out;
     if (rh == R) {
 #pragma unroll
-        for (int u = 0; u < R * n / 256; u++) {
-            const int i = tid + 256 * u;
+        for (int u = 0; u < R * n / kFftThreads; u++) {
+            const int i = tid + kFftThreads * u;
             const int rr = i / n, k = i % n;
             out[(size_t)(r0 + rr) * n + k] = stg[rr * (n + 1) + k];
         }
     } else {
-        for (int i = tid; i < rh * n; i += 256) {
+        for (int i = tid; i < rh * n; i += kFftThreads) {
             const int rr = i / n, k = i % n;
             out[(size_t)(r0 + rr) * n + k] = stg[rr * (n + 1) + k];
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_fft_pair(const FftJob* jobs, const int32_t* fblk) {
+__global__ __launch_bounds__(kFftThreads) void k_fft_pair(const FftJob* jobs, const int32_t* fblk) {
     __shared__ f32x2 ex[kFftLds];
     const int jb = ((const EP_GLOBAL int32_t*)fblk)[2 * blockIdx.x];
     if (jb < 0) return;
@@ -424,7 +428,7 @@ __global__ __launch_bounds__(256) void k_fft_pair(const FftJob* jobs, const int3
     }
 }
 
-__global__ __launch_bounds__(256) void k_fft_pass(const FftJob* jobs, const int32_t* fblk) {
+__global__ __launch_bounds__(kFftThreads) void k_fft_pass(const FftJob* jobs, const int32_t* fblk) {
     __shared__ f32x2 ex[kFftLds];
     const int jb = ((const EP_GLOBAL int32_t*)fblk)[2 * blockIdx.x];
     if (jb < 0) return;
@@ -442,7 +446,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(const FftJob* jobs, const int3
     }
 }
 
-__global__ __launch_bounds__(256) void k_fft_solve(const FftJob* jobs, const int32_t* fblk,
+__global__ __launch_bounds__(kFftThreads) void k_fft_solve(const FftJob* jobs, const int32_t* fblk,
                                                    const DesignDev* dd) {
     __shared__ f32x2 ex[kSolveEx];
     __shared__ float phi[kSolveRows];
@@ -468,18 +472,18 @@ int fft_solve_rows_per_block(int n) { return fft_solve_rows(n); }
 void launch_fft_pass(const FftJob* jobs, const int32_t* fblk, int blocks, const DesignDev* dd,
                      hipStream_t s) {
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(256), 0, s, jobs, fblk);
+    hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(kFftThreads), 0, s, jobs, fblk);
 }
 
 void launch_fft_pair(const FftJob* jobs, const int32_t* fblk, int blocks, hipStream_t s) {
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(k_fft_pair, dim3(blocks), dim3(256), 0, s, jobs, fblk);
+    hipLaunchKernelGGL(k_fft_pair, dim3(blocks), dim3(kFftThreads), 0, s, jobs, fblk);
 }
 
 void launch_fft_solve(const FftJob* jobs, const int32_t* fblk, int blocks, const DesignDev* dd,
                       hipStream_t s) {
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(k_fft_solve, dim3(blocks), dim3(256), 0, s, jobs, fblk, dd);
+    hipLaunchKernelGGL(k_fft_solve, dim3(blocks), dim3(kFftThreads), 0, s, jobs, fblk, dd);
 }
 
 }  // namespace ep

@@ -1,6 +1,6 @@
 // CPU build of the FFT engine's kernels of eisenjoch_amd/csrc/ep_fft.hip: the
 // kernels' own source (extracted into fft_kernel_extract.inc by the test)
-// runs with 256 std::threads per block and a std::barrier for __syncthreads,
+// runs with one std::thread per GPU thread of a block and a std::barrier for __syncthreads,
 // on exact-size heap buffers (AddressSanitizer reports any access outside
 // them).  Each case runs the engine's three passes on a random density, with
 // the jobs and block tables laid out as the library lays them out, and
@@ -42,10 +42,10 @@ template <typename K>
 static void run(const std::vector<int32_t>& fblk, K kern) {
     for (size_t b = 0; b < fblk.size() / 2; b++) {
         blockIdx.x = (int)b;
-        std::barrier<> bar(256);
+        std::barrier<> bar(kFftThreads);
         g_bar = &bar;
         std::vector<std::thread> th;
-        for (int t = 0; t < 256; t++) th.emplace_back([&, t] { threadIdx.x = t; kern(); });
+        for (int t = 0; t < kFftThreads; t++) th.emplace_back([&, t] { threadIdx.x = t; kern(); });
         for (auto& x : th) x.join();
     }
 }
