@@ -115,28 +115,35 @@ constexpr int kFftLds = (kFftThreads / 8) * 16 * 9;  // f32x2 elements
 constexpr int kSolveEx = (kFftThreads / 16) * 16 * 17;  // f32x2 elements
 constexpr int kSolveRows = (kFftThreads / 16) * 257;    // floats
 
+// Transform families of a call site, known when it is compiled: the DCT-II,
+// C3, and the two sine kinds (DST-III of the reference field, the IDXST
+// transform), between which the job's kind chooses at run time.
+enum FftFamily { kFamDct2 = 0, kFamC3 = 1, kFamSine = 2 };
+
 // The complex input points j = t + N2 m (m < N1) of thread t's share of one
-// row, for the transform kind: DCT-II takes v_j = x_{2j} (j < n/2) or
-// x_{2(n-1-j)+1}; the C3 kinds take e^{-i pi j/2n} (Z_j + i Z_{n-j}), Z_n = 0,
-// with Z the row (C3), the reversed row (DST-III) or (0, x_{n-1}, ..., x_1)
-// (IDXST).  row(i) returns element i of the row.  Every access is made for
-// every kind (indices wrapped into the row where the element is zeroed), so
-// no load's address depends on a branch over the kind.
-template <int N1, int N2, typename Row>
+// row: DCT-II takes v_j = x_{2j} (j < n/2) or x_{2(n-1-j)+1}; the C3 kinds
+// take e^{-i pi j/2n} (Z_j + i Z_{n-j}), Z_n = 0, with Z the row (C3), the
+// reversed row (DST-III, kind 2) or (0, x_{n-1}, ..., x_1) (IDXST, kind 3).
+// row(i) returns element i of the row.  The loads of a family are the same
+// whatever the run-time kind (indices wrapped into the row where the element
+// is zeroed), so no load's address depends on a branch.
+template <int N1, int N2, int F, typename Row>
 __device__ inline void fft_input(int kind, int t, const EP_GLOBAL f32x2* dtw, Row row, f32x2 (&a)[N1]) {
     constexpr int n = N1 * N2;
 #pragma unroll
     for (int m = 0; m < N1; m++) {
         const int j = t + N2 * m;
-        const int i0 = j < n / 2 ? 2 * j : 2 * (n - 1 - j) + 1;
-        const int ia = kind == 0 ? i0 : kind == 1 ? j : kind == 2 ? n - 1 - j : (n - j) & (n - 1);
-        const int ib = kind == 1 ? (n - j) & (n - 1) : kind == 2 ? (j - 1) & (n - 1) : j;
-        const float xa = row(ia), xb = row(ib);
-        const f32x2 c = dtw[j];
-        const float za = (kind == 3 && j == 0) ? 0.f : xa;
-        const float zb = j == 0 ? 0.f : xb;
-        const f32x2 w = cmul(c, (f32x2){za, zb});
-        a[m] = kind == 0 ? (f32x2){xa, 0.f} : w;
+        if (F == kFamDct2) {
+            a[m] = (f32x2){row(j < n / 2 ? 2 * j : 2 * (n - 1 - j) + 1), 0.f};
+        } else {
+            const int ia = F == kFamC3 ? j : kind == 2 ? n - 1 - j : (n - j) & (n - 1);
+            const int ib = F == kFamC3 ? (n - j) & (n - 1) : kind == 2 ? (j - 1) & (n - 1) : j;
+            const float xa = row(ia), xb = row(ib);
+            const f32x2 c = dtw[j];
+            const float za = (F == kFamSine && kind == 3 && j == 0) ? 0.f : xa;
+            const float zb = j == 0 ? 0.f : xb;
+            a[m] = cmul(c, (f32x2){za, zb});
+        }
     }
 }
 
@@ -153,18 +160,18 @@ __device__ inline void fft_phase1(f32x2 (&a)[N1], const EP_GLOBAL f32x2* tw, f32
 // Output index of phase-2 value (q, k2) of thread t: k = k1 + N1 k2 with
 // k1 = t + N2 q; DCT-II writes it at k, the C3 kinds at j = 2k (k < n/2) or
 // 2(n-1-k)+1.
-template <int N1, int N2>
-__device__ inline int fft_out_index(int kind, int t, int q, int k2) {
+template <int N1, int N2, int F>
+__device__ inline int fft_out_index(int t, int q, int k2) {
     constexpr int n = N1 * N2;
     const int k = t + N2 * q + N1 * k2;
-    return kind == 0 ? k : (k < n / 2 ? 2 * k : 2 * (n - 1 - k) + 1);
+    return F == kFamDct2 ? k : (k < n / 2 ? 2 * k : 2 * (n - 1 - k) + 1);
 }
 
 // Phase 2: for this thread's k1 = t + N2 q (< N1), the N2-point DFT of the
 // exchanged values; emit(q, k2, j, value) receives output k = k1 + N1 k2 of
-// the kind's real transform at its index j: DCT-II X_k = Re(e^{-i pi k/2n}
-// V_k); C3 kinds Re(w_k) / 2, signed (-1)^j for DST-III and -(-1)^j for IDXST.
-template <int N1, int N2, typename Emit>
+// the real transform at its index j: DCT-II X_k = Re(e^{-i pi k/2n} V_k); C3
+// kinds Re(w_k) / 2, signed (-1)^j for DST-III and -(-1)^j for IDXST.
+template <int N1, int N2, int F, typename Emit>
 __device__ inline void fft_phase2(const f32x2* ex, int r, int t, int kind, const EP_GLOBAL f32x2* dtw,
                                   Emit emit) {
     constexpr int PAD = N2 + 1, NQ = (N1 + N2 - 1) / N2;
@@ -178,13 +185,15 @@ __device__ inline void fft_phase2(const f32x2* ex, int r, int t, int kind, const
             dft_reg<N2>(b);
 #pragma unroll
             for (int k2 = 0; k2 < N2; k2++) {
-                const int k = k1 + N1 * k2;
-                const f32x2 c = dtw[k];
-                const int j = fft_out_index<N1, N2>(kind, t, q, k2);
-                const float x0 = c.x * b[k2].x - c.y * b[k2].y;
-                const float y = 0.5f * b[k2].x;
-                const bool neg = (kind == 2 && (j & 1)) || (kind == 3 && !(j & 1));
-                emit(q, k2, j, kind == 0 ? x0 : neg ? -y : y);
+                const int j = fft_out_index<N1, N2, F>(t, q, k2);
+                if (F == kFamDct2) {
+                    const f32x2 c = dtw[j];
+                    emit(q, k2, j, c.x * b[k2].x - c.y * b[k2].y);
+                } else {
+                    const float y = 0.5f * b[k2].x;
+                    const bool neg = F == kFamSine && ((kind == 2 && (j & 1)) || (kind == 3 && !(j & 1)));
+                    emit(q, k2, j, neg ? -y : y);
+                }
             }
         }
     }
@@ -197,15 +206,15 @@ struct FftOut {
 };
 
 // The thread's outputs into a staged row (stride n + 1 floats).
-template <int N1, int N2>
-__device__ inline void fft_stage(float* stg, int r, int t, int kind, const FftOut<N1, N2>& o) {
+template <int N1, int N2, int F>
+__device__ inline void fft_stage(float* stg, int r, int t, const FftOut<N1, N2>& o) {
     constexpr int n = N1 * N2, NQ = (N1 + N2 - 1) / N2;
 #pragma unroll
     for (int q = 0; q < NQ; q++)
         if (t + N2 * q < N1) {
 #pragma unroll
             for (int k2 = 0; k2 < N2; k2++)
-                stg[r * (n + 1) + fft_out_index<N1, N2>(kind, t, q, k2)] = o.v[q * N2 + k2];
+                stg[r * (n + 1) + fft_out_index<N1, N2, F>(t, q, k2)] = o.v[q * N2 + k2];
         }
 }
 
@@ -235,8 +244,8 @@ __device__ inline void fft_store(const float* stg, EP_GLOBAL float* out, int ld,
     }
 }
 
-// A row pass: rows r0 .. r0 + rh of the job's plane, one transform kind,
-// stored transposed (epi 0) or in row order (epi 2).
+// The first pass: rows r0 .. r0 + rh of the density, DCT-II along x, stored
+// transposed (element k of row r at out[k ld + r]).
 template <int N1, int N2>
 __device__ inline void fft_rows(const FftJob& J, int r0, f32x2* ex) {
     constexpr int n = N1 * N2;
@@ -246,25 +255,23 @@ __device__ inline void fft_rows(const FftJob& J, int r0, f32x2* ex) {
     const int t = tid % N2, r = tid / N2;
     const int rh = min(R, J.rows - r0);  // rows of this block (a power of two)
     const bool live = r < rh;
-    const int kind = J.kind;
     const EP_GLOBAL float* x = (const EP_GLOBAL float*)J.in + (size_t)(r0 + r) * n;
     const EP_GLOBAL f32x2* tw = (const EP_GLOBAL f32x2*)J.tw;
     const EP_GLOBAL f32x2* dtw = (const EP_GLOBAL f32x2*)J.dtw;
     if (live) {
         f32x2 a[N1];
-        fft_input<N1, N2>(kind, t, dtw, [&](int i) { return x[i]; }, a);
+        fft_input<N1, N2, kFamDct2>(0, t, dtw, [&](int i) { return x[i]; }, a);
         fft_phase1<N1, N2>(a, tw, ex, r, t);
     }
     __syncthreads();
     FftOut<N1, N2> o;
     if (live)
-        fft_phase2<N1, N2>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v) { o.v[q * N2 + k2] = v; });
+        fft_phase2<N1, N2, kFamDct2>(ex, r, t, 0, dtw, [&](int q, int k2, int, float v) { o.v[q * N2 + k2] = v; });
     __syncthreads();  // the exchange is read; its LDS now stages the outputs
     float* stg = (float*)ex;
-    if (live) fft_stage<N1, N2>(stg, r, t, kind, o);
+    if (live) fft_stage<N1, N2, kFamDct2>(stg, r, t, o);
     __syncthreads();
-    if (J.epi == 2) fft_store<n, R, true>(stg, (EP_GLOBAL float*)J.out, J.ld, J.ostride, r0, rh);
-    else fft_store<n, R, false>(stg, (EP_GLOBAL float*)J.out, J.ld, J.ostride, r0, rh);
+    fft_store<n, R, false>(stg, (EP_GLOBAL float*)J.out, J.ld, 1, r0, rh);
 }
 
 // The column solve (density.rs:113-153 between the two transforms): rows u
@@ -296,14 +303,14 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
     // forward DCT-II along y, the Poisson solve written into the LDS rows
     if (live) {
         f32x2 a[N1];
-        fft_input<N1, N2>(0, t, dtw, [&](int i) { return x[i]; }, a);
+        fft_input<N1, N2, kFamDct2>(0, t, dtw, [&](int i) { return x[i]; }, a);
         fft_phase1<N1, N2>(a, tw, ex, r, t);
     }
     __syncthreads();
     if (live) {
         const f32x2 fx = sx[kx];
         const float su = kx == 0 ? 0.5f * s0 : s0;
-        fft_phase2<N1, N2>(ex, r, t, 0, dtw, [&](int, int, int k, float v) {
+        fft_phase2<N1, N2, kFamDct2>(ex, r, t, 0, dtw, [&](int, int, int k, float v) {
             const f32x2 fy = sy[k];
             const float sc = k == 0 ? 0.5f * su : su;
             const float den = fx.y + fy.y;
@@ -316,29 +323,29 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
     if (live) {
         const float wu = sx[kx].x;
         f32x2 a[N1];
-        fft_input<N1, N2>(1, t, dtw, [&](int i) { return prow[i] * wu; }, a);
+        fft_input<N1, N2, kFamC3>(1, t, dtw, [&](int i) { return prow[i] * wu; }, a);
         fft_phase1<N1, N2>(a, tw, ex, r, t);
     }
     __syncthreads();
     if (live)
-        fft_phase2<N1, N2>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v) { oa.v[q * N2 + k2] = v; });
+        fft_phase2<N1, N2, kFamC3>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v) { oa.v[q * N2 + k2] = v; });
     __syncthreads();
     // Ey path: the sine kind along y of phi w_v
     const int kind = J.kind;
     if (live) {
         f32x2 a[N1];
-        fft_input<N1, N2>(kind, t, dtw, [&](int i) { return prow[i] * sy[i].x; }, a);
+        fft_input<N1, N2, kFamSine>(kind, t, dtw, [&](int i) { return prow[i] * sy[i].x; }, a);
         fft_phase1<N1, N2>(a, tw, ex, r, t);
     }
     __syncthreads();  // the Poisson rows are read: they now stage the Ex output
     float* stA = phi;
-    if (live) fft_stage<N1, N2>(stA, r, t, 1, oa);
+    if (live) fft_stage<N1, N2, kFamC3>(stA, r, t, oa);
     FftOut<N1, N2> ob;
     if (live)
-        fft_phase2<N1, N2>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v) { ob.v[q * N2 + k2] = v; });
+        fft_phase2<N1, N2, kFamSine>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v) { ob.v[q * N2 + k2] = v; });
     __syncthreads();  // the exchange is read; its LDS now stages the Ey output
     float* stB = (float*)ex;
-    if (live) fft_stage<N1, N2>(stB, r, t, kind, ob);
+    if (live) fft_stage<N1, N2, kFamSine>(stB, r, t, ob);
     __syncthreads();
     fft_store<n, R, false>(stA, (EP_GLOBAL float*)J.out, J.ld, 1, r0, rh);
     fft_store<n, R, false>(stB, (EP_GLOBAL float*)J.out2, J.ld, 1, r0, rh);
@@ -366,19 +373,19 @@ __device__ inline void fft_pair(const FftJob& J, int r0, f32x2* ex) {
     f32x2 b[N1];
     if (live) {
         f32x2 a[N1];
-        fft_input<N1, N2>(kind, t, dtw, [&](int i) { return xa[i]; }, a);
-        fft_input<N1, N2>(1, t, dtw, [&](int i) { return xb[i]; }, b);
+        fft_input<N1, N2, kFamSine>(kind, t, dtw, [&](int i) { return xa[i]; }, a);
+        fft_input<N1, N2, kFamC3>(1, t, dtw, [&](int i) { return xb[i]; }, b);
         fft_phase1<N1, N2>(a, tw, ex, r, t);
     }
     __syncthreads();
     FftOut<N1, N2> oa, ob;
     if (live)
-        fft_phase2<N1, N2>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v) { oa.v[q * N2 + k2] = v; });
+        fft_phase2<N1, N2, kFamSine>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v) { oa.v[q * N2 + k2] = v; });
     __syncthreads();
     if (live) fft_phase1<N1, N2>(b, tw, ex, r, t);
     __syncthreads();
     if (live)
-        fft_phase2<N1, N2>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v) { ob.v[q * N2 + k2] = v; });
+        fft_phase2<N1, N2, kFamC3>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v) { ob.v[q * N2 + k2] = v; });
     __syncthreads();  // the exchange is read; its LDS now stages (Ex, Ey) pairs
     f32x2* stg = ex;
     if (live) {
@@ -388,8 +395,8 @@ __device__ inline void fft_pair(const FftJob& J, int r0, f32x2* ex) {
             if (t + N2 * q < N1) {
 #pragma unroll
                 for (int k2 = 0; k2 < N2; k2++) {
-                    stg[r * (n + 1) + fft_out_index<N1, N2>(kind, t, q, k2)].x = oa.v[q * N2 + k2];
-                    stg[r * (n + 1) + fft_out_index<N1, N2>(1, t, q, k2)].y = ob.v[q * N2 + k2];
+                    const int j = fft_out_index<N1, N2, kFamC3>(t, q, k2);
+                    stg[r * (n + 1) + j] = (f32x2){oa.v[q * N2 + k2], ob.v[q * N2 + k2]};
                 }
             }
     }
