@@ -1,8 +1,8 @@
 // Internal layout shared by the HIP kernels and the host runtime.
 //
 // One resident design owns one device arena.  Every per-iteration kernel
-// covers a whole batch of designs: a block finds its design by binary search
-// over a prefix of per-design block counts, and skips the design when its
+// covers a whole batch of designs: a block finds its design and its block
+// index inside the design in a per-grid block table, and skips the design when its
 // state says it is done.  All control scalars are FP64 and live on the
 // device, so a batch advances without host round trips.
 #pragma once
@@ -24,6 +24,7 @@ constexpr int kScatCells = 1024;     // cells per scatter block
 constexpr int kScatBins = 6144;      // LDS accumulation window of a scatter block (bins)
 constexpr int kHfStat = 16;      // per high-fanout net: 4 extremes, 8 sums, weight, pad
 constexpr int kXcds = 8;          // MI355X: blocks are dealt to the 8 XCDs in turn
+constexpr int kRefBits = 13;      // design bits of a block-table entry (entry = design | block << 13)
 constexpr int kCellSlots = 12;    // slots of a cell held at a fixed stride (cs8)
 constexpr int kWaUnits = 3;       // 64-slot net units per wave of the WA kernel
 // blocks (four waves) of the WA kernel for a design of n_wa_waves units
@@ -208,8 +209,9 @@ struct BatchDev {
     const int32_t* pfx_hf;     // one block per high-degree net
     const int32_t* pfx_scat;   // atomic scatter blocks (kScatCells cells each)
     const int32_t* pfx_tile;   // pull scatter blocks (one per 8x8-bin tile)
-    // design of each block of the same six grids (one load at block start;
-    // null in a one-design view, where the design is 0)
+    // each block's (design, block of the design) in the same six grids, packed
+    // as design | block << kRefBits, or -1 for a padding block (one load at
+    // block start; null in a one-design view, where the prefix gives both)
     const int32_t* blk_cell;
     const int32_t* blk_bin;
     const int32_t* blk_wa;

@@ -1,8 +1,9 @@
 // Per-iteration kernels of the electrostatic placer on CDNA4 (gfx950).
 //
 // Reference loop: crates/nextpnr/src/placer/electro_place/mod.rs:104-197.
-// Each kernel covers a batch of designs; blocks find their design through a
-// prefix of per-design block counts and skip designs whose state is done.
+// Each kernel covers a batch of designs; blocks find their design and their
+// block of it in a block table (the host deals each design's blocks to one
+// XCD, or over all of them) and skip designs whose state is done.
 //
 //   k_wa          WA wirelength gradient (wa.rs:51-101, common.rs:205-265):
 //                 one group of G lanes (G = 2..64, a power of two) per net,
@@ -58,10 +59,22 @@ __device__ inline int find_blk(const int32_t* pfx, int nd, int blk) {
     return lo;
 }
 
-// The design of this block: from the block's table entry, or (one-design
-// views, no table) by search.
-__device__ inline int design_of(const int32_t* tab, const int32_t* pfx, int nd, int blk) {
-    return tab ? tab[blk] : find_blk(pfx, nd, blk);
+// The design of this block and the block's index inside the design: from
+// the block's table entry (the design in the low kRefBits bits, the block
+// above them; -1 for a block that only pads the XCDs to equal length), or
+// (one-design views, no table) by search over the block prefix.  A block's
+// work depends on this pair only, so any order of the table gives the same
+// results bit for bit.
+struct BlockRef {
+    int d, blk;
+};
+__device__ inline BlockRef block_ref(const int32_t* tab, const int32_t* pfx, int nd, int p) {
+    if (!tab) {
+        const int d = find_blk(pfx, nd, p);
+        return {d, p - pfx[d]};
+    }
+    const int32_t e = tab[p];
+    return {e < 0 ? -1 : (int)(e & ((1 << kRefBits) - 1)), (int)(e >> kRefBits)};
 }
 
 template <typename T>
@@ -245,7 +258,9 @@ __device__ inline void wa_unit(const WaArgs<T>& A, int G, int lane, uint32_t c, 
 // dependent load chains in flight and never waits on its own stores.
 template <typename T, bool FIX, bool HPWL>
 __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
-    const int d = design_of(b.blk_wa, b.pfx_wa, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_wa, b.pfx_wa, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
     const DevState* st = D.st;
     if (HPWL) {
@@ -256,7 +271,7 @@ __global__ __launch_bounds__(256) void k_wa(BatchDev b, int final_pass) {
     // the wave index through readfirstlane: the unit, its bucket and group
     // size are then scalars and the descriptor fields they select scalar loads
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int w0 = ((blockIdx.x - b.pfx_wa[d]) * 4 + wv) * kWaUnits;
+    const int w0 = (ref.blk * 4 + wv) * kWaUnits;
     if (w0 >= D.n_wa_waves) return;  // uniform per wave; no block barrier below
     const int lane = threadIdx.x & 63;
     const int nu = min(kWaUnits, D.n_wa_waves - w0);
@@ -327,7 +342,9 @@ template <typename T, bool FIX, bool HPWL>
 __global__ __launch_bounds__(kHfThreads) void k_wa_hf(BatchDev b, int final_pass) {
     __shared__ double sh[kHfWaves * 8];
     __shared__ double tot[8];
-    const int d = design_of(b.blk_hf, b.pfx_hf, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_hf, b.pfx_hf, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
     const DevState* st = D.st;
     if (HPWL) {
@@ -335,7 +352,7 @@ __global__ __launch_bounds__(kHfThreads) void k_wa_hf(BatchDev b, int final_pass
     } else if (st->done) {
         return;
     }
-    const int e = blockIdx.x - b.pfx_hf[d];
+    const int e = ref.blk;
     if (e >= D.n_hf) return;
     const uint32_t p0 = D.hf_off[e], p1 = D.hf_off[e + 1];
     const bool cached = p1 - p0 <= (uint32_t)(kHfThreads * kHfCache);
@@ -492,10 +509,12 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
     constexpr int kCap = FIX ? kScatBins : kScatBins / 2;
     __shared__ Acc win[kScatBins];
     __shared__ int box[4];
-    const int d = design_of(b.blk_scat, b.pfx_scat, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_scat, b.pfx_scat, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
-    const int c0 = (blockIdx.x - b.pfx_scat[d]) * kScatCells;
+    const int c0 = ref.blk * kScatCells;
     const uint32_t* PERM = D.perm[(D.st->it + 1) & 1];
     typedef typename Vec2<T>::type V;
     const V* LXY = (const V*)D.look;
@@ -567,10 +586,12 @@ __global__ __launch_bounds__(256) void k_scatter(BatchDev b) {
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_dens_final(BatchDev b) {
     __shared__ double sh[4 * kBinParts];
-    const int d = design_of(b.blk_bin, b.pfx_bin, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_bin, b.pfx_bin, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
-    const int blk = blockIdx.x - b.pfx_bin[d];
+    const int blk = ref.blk;
     const int i = blk * kBlock + threadIdx.x;
     const int nb = D.Bx * D.By;
     double v = 0.0;
@@ -653,10 +674,12 @@ __device__ inline uint32_t wave_agg_add(uint32_t* ctr, uint32_t key, bool valid)
 // contiguous arrays and places each cell at tstart[key] + offset.
 template <typename T>
 __global__ __launch_bounds__(256) void k_tile_count(BatchDev b) {
-    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
-    const int k = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    const int k = ref.blk * kBlock + threadIdx.x;
     const bool valid = k < D.n_mov;
     uint32_t key = 0u;
     if (valid) {
@@ -739,10 +762,12 @@ __global__ __launch_bounds__(256) void k_tile_scan(BatchDev b) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_tile_fill(BatchDev b) {
-    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
-    const int k = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    const int k = ref.blk * kBlock + threadIdx.x;
     const bool valid = k < D.n_mov;
     const int par = D.st->it & 1;
     const uint32_t i = valid ? D.perm[par][k] : 0u;
@@ -772,10 +797,12 @@ __global__ __launch_bounds__(256) void k_scatter_pull(BatchDev b) {
     // FAST: one fixed-point word per bin; FP64: the whole words, then the fractions
     __shared__ Acc win[(FIX ? 1 : 2) * 32 * 32];
     __shared__ double sh[4 * kBinParts];
-    const int d = design_of(b.blk_tile, b.pfx_tile, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_tile, b.pfx_tile, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
-    const int t = blockIdx.x - b.pfx_tile[d];
+    const int t = ref.blk;
     const int stx = (D.tiles_x + 3) / 4;
     const int sx = t % stx, sy = t / stx;
     const int x0 = sx * 32, y0 = sy * 32;
@@ -942,10 +969,12 @@ __device__ inline void gwl_of(const DesignDev& D, int i, double& gx, double& gy)
 // kernels, beside the density chain.
 template <typename T>
 __global__ __launch_bounds__(256) void k_wl_cell(BatchDev b) {
-    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
     if (D.st->done) return;
-    const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    const int i = ref.blk * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
     double2 v;
     gwl_of<T>(D, i, v.x, v.y);
@@ -960,12 +989,14 @@ __global__ __launch_bounds__(256) void k_wl_cell(BatchDev b) {
 template <typename T, bool FIX, int MODE>
 __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, double* ey_out) {
     __shared__ double sh[4 * 4];
-    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
     const DevState* st = D.st;
     if (st->done) return;
     if (MODE == 1 && !st->need_fix) return;
-    const int blk = blockIdx.x - b.pfx_cell[d];
+    const int blk = ref.blk;
     const int i = blk * kBlock + threadIdx.x;
     const int it = st->it, par = it & 1;
     double part[4] = {0, 0, 0, 0};
@@ -1252,11 +1283,13 @@ __global__ __launch_bounds__(256) void k_ctrl(BatchDev b, int which) {
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_update(BatchDev b) {
     __shared__ double sh[4 * 2];
-    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
     const DevState* st = D.st;
     if (!st->run_update) return;
-    const int blk = blockIdx.x - b.pfx_cell[d];
+    const int blk = ref.blk;
     const int i = blk * kBlock + threadIdx.x;
     const int par = (st->it - 1) & 1;
     double sq[2] = {0, 0};
@@ -1310,9 +1343,11 @@ __global__ __launch_bounds__(256) void k_update(BatchDev b) {
 // nesterov.rs:114-118; first look-ahead has zero momentum, nesterov.rs:75-82).
 template <typename T, bool FIX>
 __global__ __launch_bounds__(256) void k_reset(BatchDev b) {
-    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
-    const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    const int i = ref.blk * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
     if (i == 0) D.st->t_start = wall_clock64();  // the placement's start, for its latency
     D.perm[0][i] = (uint32_t)i;
@@ -1335,9 +1370,11 @@ __global__ __launch_bounds__(256) void k_reset(BatchDev b) {
 // Evaluation helper: the WA gradient per cell (the gather's slot reduction).
 template <typename T>
 __global__ __launch_bounds__(256) void k_wa_reduce(BatchDev b, double* gx, double* gy) {
-    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
-    const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    const int i = ref.blk * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
     double a, c;
     gwl_of<T>(D, i, a, c);
@@ -1347,9 +1384,11 @@ __global__ __launch_bounds__(256) void k_wa_reduce(BatchDev b, double* gx, doubl
 
 // Evaluation helper: integer bin ranges per cell.
 __global__ __launch_bounds__(256) void k_bin_ranges(BatchDev b, int32_t* out) {
-    const int d = design_of(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    const BlockRef ref = block_ref(b.blk_cell, b.pfx_cell, b.nd, blockIdx.x);
+    if (ref.d < 0) return;
+    const int d = ref.d;
     const DesignDev& D = b.d[d];
-    const int i = (blockIdx.x - b.pfx_cell[d]) * kBlock + threadIdx.x;
+    const int i = ref.blk * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
     double cx, cy;
     if (D.prec == EP_PREC_FP64) {

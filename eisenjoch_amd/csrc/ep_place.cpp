@@ -75,6 +75,62 @@ static std::vector<int> engines_for_shape(int Bx, int By) {
     return e;
 }
 
+// Block table of one grid (P: the per-design block prefix).  Entry p is
+// design | (block << kRefBits), or -1 for a block that pads the XCDs to
+// equal length.  The dispatcher deals consecutive blocks to the kXcds XCDs
+// in turn, so position p runs on XCD p % kXcds: a design with xcd_of[d] >= 0
+// has all its blocks at that XCD's positions (its data is then fetched into
+// one L2), a design with xcd_of[d] < 0 has its blocks dealt over every XCD.
+static void build_block_table(const std::vector<int32_t>& P, int nd, const std::vector<int>& xcd_of,
+                              std::vector<int32_t>& tab) {
+    std::vector<std::vector<int32_t>> lists(kXcds);
+    int turn = 0;
+    for (int d = 0; d < nd; d++) {
+        const int n = P[d + 1] - P[d];
+        for (int j = 0; j < n; j++) {
+            const int x = d < (int)xcd_of.size() && xcd_of[d] >= 0 ? xcd_of[d] % kXcds : (turn++ % kXcds);
+            lists[x].push_back((int32_t)d | ((int32_t)j << kRefBits));
+        }
+    }
+    size_t len = 0;
+    for (auto& l : lists) len = std::max(len, l.size());
+    tab.assign(std::max<size_t>(1, len * kXcds), -1);
+    for (int x = 0; x < kXcds; x++)
+        for (size_t q = 0; q < lists[x].size(); q++) tab[q * kXcds + x] = lists[x][q];
+    if (P[nd] == 0) tab.assign(1, -1);
+}
+
+// XCD of each design: designs whose work (cells plus pins) is at most an
+// XCD's share of the batch go whole to the least loaded XCD, largest first,
+// so the random reads of their kernels (pin positions, slot values, the
+// tile count's positions) are served from that XCD's L2; larger designs are
+// dealt over every XCD.  The FFT passes keep every design on every XCD:
+// their work follows the bin count, which this weight does not balance.
+static std::vector<int> assign_xcds(const std::vector<Design*>& ds) {
+    const int nd = (int)ds.size();
+    std::vector<int> xcd(nd, -1);
+    std::vector<double> w(nd, 0.0);
+    double tot = 0.0;
+    for (int d = 0; d < nd; d++) {
+        w[d] = (double)ds[d]->dev.n_mov + (double)(ds[d]->cs_count + ds[d]->hf_pin_count);
+        tot += w[d];
+    }
+    std::vector<int> order(nd);
+    for (int d = 0; d < nd; d++) order[d] = d;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return w[a] > w[b2]; });
+    std::vector<double> load(kXcds, 0.0);
+    for (int d : order) {
+        if (w[d] > tot / kXcds) {
+            for (double& l : load) l += w[d] / kXcds;
+            continue;
+        }
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += w[d];
+        xcd[d] = x;
+    }
+    return xcd;
+}
+
 // Per pass: (A kind, A matrix or buffer, B kind, B matrix or buffer, epilogue, out, out2).
 // The device image (descriptors, prefixes, jobs, block tables) goes into a
 // fresh allocation, or, with ext_mem, into that caller-owned device buffer of
@@ -216,12 +272,6 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         jobs[3][eng].push_back(i2a);
         jobs[3][eng].push_back(i2b);
     }
-    b.cell_blocks = pc[nd];
-    b.bin_blocks = pb[nd];
-    b.wa_blocks = pw[nd];
-    b.hf_blocks = ph[nd];
-    b.scat_blocks = ps[nd];
-    b.tile_blocks = pt[nd];
     // device image: descriptors, prefixes, jobs
     size_t bytes = align_up(sizeof(DesignDev) * nd) + 6 * align_up(sizeof(int32_t) * (nd + 1)) + align_up(4);
     // Block map of each pass: the dispatcher deals consecutive blocks to the
@@ -303,7 +353,9 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             bytes += align_up(sizeof(GemmJob) * std::max<size_t>(1, jobs[p][e].size()));
             bytes += align_up(sizeof(int32_t) * std::max<size_t>(2, bjob[p][e].size()));
         }
-    // FFT engine: (job, first row) of every block of a pass
+    // FFT engine: (job, first row) of every block of a pass, dealt over
+    // every XCD in turn (a pass's work follows the bin count of each design,
+    // which the per-design XCD placement of the grids does not balance)
     std::vector<int32_t> fblk[4];
     for (int p = 0; p < 4; p++) {
         for (size_t q = 0; q < fjobs[p].size(); q++)
@@ -317,14 +369,19 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
         bytes += align_up(sizeof(int32_t) * std::max<size_t>(2, fblk[p].size()));
     }
     bytes += align_up(sizeof(PlaneSum) * std::max<size_t>(1, sums.size()));
-    // block -> design tables of the six grids
+    // block tables of the six grids
     const std::vector<int32_t>* pfxs[6] = {&pc, &pb, &pw, &ph, &ps, &pt};
+    int* grid[6] = {&b.cell_blocks, &b.bin_blocks, &b.wa_blocks, &b.hf_blocks, &b.scat_blocks, &b.tile_blocks};
+    if (nd > (1 << kRefBits)) return fail(c, EP_ERR_INVALID_ARG, "too many designs for the block tables");
+    const std::vector<int> xcd_of = assign_xcds(ds);
     std::vector<int32_t> blk[6];
     for (int g = 0; g < 6; g++) {
         const std::vector<int32_t>& P = *pfxs[g];
-        blk[g].resize(std::max(1, P[nd]), 0);
         for (int i = 0; i < nd; i++)
-            std::fill(blk[g].begin() + P[i], blk[g].begin() + P[i + 1], i);
+            if (P[i + 1] - P[i] > (1 << (31 - kRefBits)) - 1)
+                return fail(c, EP_ERR_INVALID_ARG, "design too large for the block tables");
+        build_block_table(P, nd, xcd_of, blk[g]);
+        *grid[g] = P[nd] > 0 ? (int)blk[g].size() : 0;
         bytes += align_up(sizeof(int32_t) * blk[g].size());
     }
     if (ext_mem) {
