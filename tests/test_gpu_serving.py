@@ -129,7 +129,19 @@ def test_serving_failure_ends_every_ticket(monkeypatch):
         log_case("serving: forced failure after 3 chunks")
         srv = ep.Server(dev, max_slots=2, iters_per_sync=5)
         try:
-            tickets = [srv.submit(d, CFG) for d in ds]
+            # the serving thread runs while the designs are submitted, so the
+            # failure may come before the last submissions: those are refused
+            # with it, and every one accepted before it ends with it
+            tickets, refused = [], 0
+            for d in ds:
+                try:
+                    tickets.append(srv.submit(d, CFG))
+                except ep.DeviceError as e:
+                    assert "forced serving failure" in str(e)
+                    refused += 1
+                else:
+                    assert refused == 0  # no submission is accepted after a refusal
+            assert tickets
             res = wait(srv, tickets, timeout=60.0)  # nothing hangs
             # 15 iterations ran before the failure: no design can have stopped (min 20)
             assert all(res[t].status == ep._lib.EP_ERR_DEVICE for t in tickets)
@@ -137,7 +149,7 @@ def test_serving_failure_ends_every_ticket(monkeypatch):
                 srv.submit(ds[0], CFG)
             st = srv.stats()
             assert st["failed"] == ep._lib.EP_ERR_DEVICE and st["chunks"] == 3
-            assert st["finished"] == 0 and st["admitted"] == 2
+            assert st["finished"] == 0 and 1 <= st["admitted"] <= min(2, len(tickets))
             with pytest.raises(ep.PlacerError):
                 srv.positions(tickets[0])
             for t in tickets:
