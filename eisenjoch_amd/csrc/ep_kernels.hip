@@ -1031,8 +1031,8 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
                 T e0[9], e1[9];
                 if constexpr (std::is_same<T, float>::value) {
                     // FAST path: a row's three pairs in two loads, the first
-                    // two pairs as one 16-byte load (6 addresses per cell
-                    // instead of 9: the gather is bound by the scattered
+                    // two pairs as one 16-byte load (at most 6 addresses per
+                    // cell instead of 9: the gather is bound by the scattered
                     // addresses the L1 takes per clock).  Every address is
                     // inside the field: rows past the plane are clamped to
                     // its last row and the pair to the row's last two bins
@@ -1044,8 +1044,13 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
 #pragma unroll
                     for (int j = 0; j < 3; j++) {
                         const size_t row = (size_t)min(r.by0 + j, D.By - 1) * D.Bx;
-                        const f4a8 pr = *(const f4a8*)&E[row + i0];
-                        const V t = E[row + i2];
+                        // the third row and column only where the footprint reaches them
+                        f4a8 pr = {0.f, 0.f, 0.f, 0.f};
+                        V t = {0.f, 0.f};
+                        if (j < 2 || r.by0 + 2 < r.by1) {
+                            pr = *(const f4a8*)&E[row + i0];
+                            if (r.bx0 + 2 < r.bx1) t = E[row + i2];
+                        }
                         e0[3 * j] = hi ? pr.z : pr.x;
                         e1[3 * j] = hi ? pr.w : pr.y;
                         e0[3 * j + 1] = pr.z;
