@@ -298,7 +298,7 @@ KERNEL_GROUPS = {
            "note": "second stream: overlaps the tile sort, scatter and transforms"},
     "dct (4 transform passes, BF16 hi+lo MFMA)": {
         "phases": ["dct_f1", "dct_f2", "dct_i1", "dct_i2"], "bound": "mfma", "work": "dct_flops",
-        "kernels": {"ep::k_gemm_mfma<": 4}},
+        "kernels": {"ep::k_gemm_mfma<32, 2, true, false>": 2, "ep::k_gemm_mfma<32, 2, false, false>": 2}},
 }
 # The transforms on the FFT engine (every shape of the batch on it): three
 # passes, priced by the bytes of the planes they read and write.
