@@ -26,6 +26,8 @@
 // value stays FP32; the twiddles are FP64 values rounded once.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "ep_internal.h"
 
 namespace ep {
@@ -148,13 +150,16 @@ __device__ inline void fft_input(int kind, int t, const EP_GLOBAL f32x2* dtw, Ro
 }
 
 // Phase 1: the N1-point DFT of the thread's points, times e^{-2 pi i t k1/n},
-// into the exchange (row r, k1, t).
+// into the exchange (row r, k1, t).  The twiddle table is laid out for the
+// split, e^{-2 pi i t k1/n} at k1 N2 + t, so for each k1 the lanes of a wave
+// read consecutive entries (one or two cache lines, where t k1 order would
+// touch one line per lane).
 template <int N1, int N2>
 __device__ inline void fft_phase1(f32x2 (&a)[N1], const EP_GLOBAL f32x2* tw, f32x2* ex, int r, int t) {
     constexpr int PAD = N2 + 1;
     dft_reg<N1>(a);
 #pragma unroll
-    for (int k1 = 0; k1 < N1; k1++) ex[(r * N1 + k1) * PAD + t] = cmul(a[k1], tw[t * k1]);
+    for (int k1 = 0; k1 < N1; k1++) ex[(r * N1 + k1) * PAD + t] = cmul(a[k1], tw[k1 * N2 + t]);
 }
 
 // Output index of phase-2 value (q, k2) of thread t: k = k1 + N1 k2 with
@@ -475,6 +480,17 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_solve(const FftJob* jobs, c
 
 int fft_rows_per_block(int n) { return fft_block_rows(n); }
 int fft_solve_rows_per_block(int n) { return fft_solve_rows(n); }
+
+void fft_twiddle_table(int n, bool solve, float* out) {
+    const int n2 = solve ? fft_n2_solve(n) : fft_n2(n), n1 = n / n2;
+    const double pi = 3.14159265358979323846;
+    for (int k1 = 0; k1 < n1; k1++)
+        for (int t = 0; t < n2; t++) {
+            const double a = 2.0 * pi * (double)(t * k1) / n;
+            out[2 * (k1 * n2 + t)] = (float)std::cos(a);
+            out[2 * (k1 * n2 + t) + 1] = (float)-std::sin(a);
+        }
+}
 
 void launch_fft_pass(const FftJob* jobs, const int32_t* fblk, int blocks, const DesignDev* dd,
                      hipStream_t s) {

@@ -190,7 +190,7 @@ struct FftJob {
     const void* in2;     // the last pass: the second component's plane
     void* out;
     void* out2;
-    const void* tw;      // float2[n]: exp(-2 pi i t / n)
+    const void* tw;      // float2[n]: exp(-2 pi i t k1 / n) at k1 N2 + t for the pass's split n = N1 N2
     const void* dtw;     // float2[n]: exp(-i pi k / 2n)
     const DevState* st;  // a block of a done design returns at once
     int32_t n, rows, rpb, kind, epi, ld, ostride, design;
@@ -261,6 +261,9 @@ void launch_fft_pass(const FftJob* jobs_dev, const int32_t* fblk, int blocks, co
 void launch_fft_solve(const FftJob* jobs_dev, const int32_t* fblk, int blocks, const DesignDev* dd,
                       hipStream_t s);
 int fft_solve_rows_per_block(int n);
+// The twiddles e^{-2 pi i t k1 / n} of the row passes' split of n (or the
+// column solve's) into out (float2[n]), at k1 N2 + t.
+void fft_twiddle_table(int n, bool solve, float* out);
 // The last pass, both components per block (pair jobs).
 void launch_fft_pair(const FftJob* jobs_dev, const int32_t* fblk, int blocks, hipStream_t s);
 

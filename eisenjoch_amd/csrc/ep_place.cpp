@@ -224,21 +224,21 @@ int batch_build_cfgs(Ctx* c, const std::vector<Design*>& ds, const std::vector<c
             // (the sine kind for Ex, C3 for Ey) into the field in row order
             const int sk = cfgs[i]->field_mode == EP_FIELD_REFERENCE ? 2 : 3;
             auto fj = [&](const void* in, void* out, void* out2, const Transform* t, int n, int rows,
-                          int kind, int epi, int ld, int os, int rpb) {
+                          int kind, int epi, int ld, int os, int rpb, bool solve) {
                 FftJob F;
                 memset(&F, 0, sizeof(F));
                 F.in = in; F.out = out; F.out2 = out2;
-                F.tw = t->fft_tw; F.dtw = t->dct_tw; F.st = D.dev.st;
+                F.tw = solve ? t->fft_tw_solve : t->fft_tw; F.dtw = t->dct_tw; F.st = D.dev.st;
                 F.n = n; F.rows = rows; F.rpb = std::min(rows, rpb);
                 F.kind = kind; F.epi = epi; F.ld = ld; F.ostride = os; F.design = i;
                 return F;
             };
             fjobs[0].push_back(fj(D.dev.dens, D.dev.tT, nullptr, tx, Bx, By, 0, 0, By, 1,
-                                  fft_rows_per_block(Bx)));
+                                  fft_rows_per_block(Bx), false));
             fjobs[1].push_back(fj(D.dev.tT, D.dev.uT[0], D.dev.uT[1], ty, By, Bx, sk, 0, Bx, 1,
-                                  fft_solve_rows_per_block(By)));
+                                  fft_solve_rows_per_block(By), true));
             FftJob last = fj(D.dev.uT[0], D.dev.field, nullptr, tx, Bx, By, sk, 2, Bx, 2,
-                             fft_rows_per_block(Bx));
+                             fft_rows_per_block(Bx), false);
             last.in2 = D.dev.uT[1];
             fjobs[3].push_back(last);
             continue;

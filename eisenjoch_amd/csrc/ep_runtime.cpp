@@ -102,7 +102,7 @@ int get_transform(Ctx* c, int n, int mode, Transform** out) {
     t->mode = mode;
     const size_t nn = (size_t)n * n;
     const size_t bytes = 3 * align_up(nn * 8) + 3 * align_up(nn * 4) + 6 * align_up(nn * 2) +
-                         2 * align_up((size_t)n * 8);
+                         3 * align_up((size_t)n * 8);
     HIPCHK(c, hipMalloc(&t->mem, bytes));
     char* p = (char*)t->mem;
     std::vector<double> m;
@@ -127,16 +127,18 @@ int get_transform(Ctx* c, int n, int mode, Transform** out) {
     // FFT engine twiddles, computed in FP64 and rounded once
     {
         const double pi = 3.14159265358979323846;
-        std::vector<float> tw(2 * (size_t)n), dtw(2 * (size_t)n);
+        std::vector<float> tw(2 * (size_t)n), tws(2 * (size_t)n), dtw(2 * (size_t)n);
+        fft_twiddle_table(n, false, tw.data());
+        fft_twiddle_table(n, true, tws.data());
         for (int k = 0; k < n; k++) {
-            tw[2 * k] = (float)std::cos(2.0 * pi * k / n);
-            tw[2 * k + 1] = (float)-std::sin(2.0 * pi * k / n);
             dtw[2 * k] = (float)std::cos(pi * k / (2.0 * n));
             dtw[2 * k + 1] = (float)-std::sin(pi * k / (2.0 * n));
         }
         t->fft_tw = (float*)p; p += align_up((size_t)n * 8);
+        t->fft_tw_solve = (float*)p; p += align_up((size_t)n * 8);
         t->dct_tw = (float*)p; p += align_up((size_t)n * 8);
         HIPCHK(c, hipMemcpy(t->fft_tw, tw.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(t->fft_tw_solve, tws.data(), (size_t)n * 8, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(t->dct_tw, dtw.data(), (size_t)n * 8, hipMemcpyHostToDevice));
     }
     c->transform_bytes += bytes;

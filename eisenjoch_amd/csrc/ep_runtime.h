@@ -37,7 +37,8 @@ struct Transform {
     float* f[3] = {nullptr, nullptr, nullptr};
     void* hi[3] = {nullptr, nullptr, nullptr};
     void* lo[3] = {nullptr, nullptr, nullptr};
-    float* fft_tw = nullptr;  // float2[n]: exp(-2 pi i t / n)
+    float* fft_tw = nullptr;        // float2[n]: twiddles of the row passes' split (fft_twiddle_table)
+    float* fft_tw_solve = nullptr;  // float2[n]: twiddles of the column solve's split
     float* dct_tw = nullptr;  // float2[n]: exp(-i pi k / 2n)
 };
 

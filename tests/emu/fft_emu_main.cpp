@@ -61,10 +61,15 @@ int main(int argc, char** argv) {
     if (argc < 4) return 2;
     const int Bx = atoi(argv[1]), By = atoi(argv[2]), mode = atoi(argv[3]);
     const double pi = 3.14159265358979323846;
-    auto tables = [&](int n, std::vector<float>& tw, std::vector<float>& dtw) {
+    // twiddles e^{-2 pi i t k1 / n} at k1 N2 + t for the pass's split n = N1 N2
+    auto tables = [&](int n, int n2, std::vector<float>& tw, std::vector<float>& dtw) {
         tw.resize(2 * n); dtw.resize(2 * n);
+        for (int k1 = 0; k1 < n / n2; k1++)
+            for (int t = 0; t < n2; t++) {
+                const double a = 2 * pi * (double)(t * k1) / n;
+                tw[2 * (k1 * n2 + t)] = (float)std::cos(a); tw[2 * (k1 * n2 + t) + 1] = (float)-std::sin(a);
+            }
         for (int k = 0; k < n; k++) {
-            tw[2 * k] = (float)std::cos(2 * pi * k / n); tw[2 * k + 1] = (float)-std::sin(2 * pi * k / n);
             dtw[2 * k] = (float)std::cos(pi * k / (2.0 * n)); dtw[2 * k + 1] = (float)-std::sin(pi * k / (2.0 * n));
         }
     };
@@ -74,8 +79,8 @@ int main(int argc, char** argv) {
         return s;
     };
     std::vector<float> twx, dtwx, twy, dtwy;
-    tables(Bx, twx, dtwx);
-    tables(By, twy, dtwy);
+    tables(Bx, fft_n2(Bx), twx, dtwx);
+    tables(By, fft_n2_solve(By), twy, dtwy);
     std::vector<float> sx = spec(Bx), sy = spec(By);
     const size_t NB = (size_t)Bx * By;
     std::vector<float> dens(NB), tT(NB), u0(NB), u1(NB), field(2 * NB, 0.f);

@@ -23,8 +23,8 @@ FFT = 5
 
 
 @pytest.mark.parametrize("mode", ["idxst", "reference"])
-@pytest.mark.parametrize("Bx,By", [(16, 16), (16, 32), (64, 64), (128, 64), (32, 256), (256, 256),
-                                   (512, 512), (512, 16)])
+@pytest.mark.parametrize("Bx,By", [(16, 16), (16, 32), (64, 64), (128, 64), (64, 128), (32, 256),
+                                   (256, 256), (512, 512), (512, 16)])
 def test_fft_engine_gives_the_field(device, mode, Bx, By):
     W = max(Bx, By)
     d = ep.generate(64, 64, W, n_fixed=0, Bx=Bx, By=By, seed=3)
