@@ -173,9 +173,10 @@ __device__ inline int fft_out_index(int t, int q, int k2) {
 }
 
 // Phase 2: for this thread's k1 = t + N2 q (< N1), the N2-point DFT of the
-// exchanged values; emit(q, k2, j, value) receives output k = k1 + N1 k2 of
-// the real transform at its index j: DCT-II X_k = Re(e^{-i pi k/2n} V_k); C3
-// kinds Re(w_k) / 2, signed (-1)^j for DST-III and -(-1)^j for IDXST.
+// exchanged values; emit(q, k2, j, value, c) receives output k = k1 + N1 k2 of
+// the real transform at its index j: DCT-II X_k = Re(c V_k) with
+// c = e^{-i pi k/2n} (passed on); C3 kinds Re(w_k) / 2, signed (-1)^j for
+// DST-III and -(-1)^j for IDXST (c = 0).
 template <int N1, int N2, int F, typename Emit>
 __device__ inline void fft_phase2(const f32x2* ex, int r, int t, int kind, const EP_GLOBAL f32x2* dtw,
                                   Emit emit) {
@@ -193,11 +194,11 @@ __device__ inline void fft_phase2(const f32x2* ex, int r, int t, int kind, const
                 const int j = fft_out_index<N1, N2, F>(t, q, k2);
                 if (F == kFamDct2) {
                     const f32x2 c = dtw[j];
-                    emit(q, k2, j, c.x * b[k2].x - c.y * b[k2].y);
+                    emit(q, k2, j, c.x * b[k2].x - c.y * b[k2].y, c);
                 } else {
                     const float y = 0.5f * b[k2].x;
                     const bool neg = F == kFamSine && ((kind == 2 && (j & 1)) || (kind == 3 && !(j & 1)));
-                    emit(q, k2, j, neg ? -y : y);
+                    emit(q, k2, j, neg ? -y : y, (f32x2){0.f, 0.f});
                 }
             }
         }
@@ -271,7 +272,7 @@ __device__ inline void fft_rows(const FftJob& J, int r0, f32x2* ex) {
     __syncthreads();
     FftOut<N1, N2> o;
     if (live)
-        fft_phase2<N1, N2, kFamDct2>(ex, r, t, 0, dtw, [&](int q, int k2, int, float v) { o.v[q * N2 + k2] = v; });
+        fft_phase2<N1, N2, kFamDct2>(ex, r, t, 0, dtw, [&](int q, int k2, int, float v, f32x2) { o.v[q * N2 + k2] = v; });
     __syncthreads();  // the exchange is read; its LDS now stages the outputs
     float* stg = (float*)ex;
     if (live) fft_stage<N1, N2, kFamDct2>(stg, r, t, o);
@@ -286,6 +287,14 @@ __device__ inline void fft_rows(const FftJob& J, int r0, f32x2* ex) {
 // transforms along y: C3 of phi w_u (the Ex path) and the job's sine kind of
 // phi w_v (the Ey path), stored transposed into out (Ex) and out2 (Ey).  The
 // inverse along x then finishes each component in a row pass.
+//
+// The factors along y are not loaded: l_v = 2 (1 - cos(pi v / n)) =
+// 4 sin^2(pi v / 2n) comes from the DCT-II factor of the same output,
+// e^{-i pi v/2n}, which phase 2 loads anyway (within two ulp of the table
+// value), and w_v = pi v / n is computed as the upload computes it (n a power
+// of two, so the quotient is exact and the value the same bit for bit).  The
+// quotient takes the hardware reciprocal (1 ulp): a division per coefficient
+// doubled the solve's time.
 template <int N1, int N2>
 __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f32x2* ex, float* phi) {
     constexpr int n = N1 * N2;
@@ -302,7 +311,6 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
     const EP_GLOBAL f32x2* dtw = (const EP_GLOBAL f32x2*)J.dtw;
     const DesignDev& D = dd[J.design];
     const EP_GLOBAL f32x2* sx = (const EP_GLOBAL f32x2*)D.spec_x;
-    const EP_GLOBAL f32x2* sy = (const EP_GLOBAL f32x2*)D.spec_y;
     const float s0 = 4.0f / ((float)D.Bx * (float)D.By);
     float* prow = phi + r * (n + 1);
     // forward DCT-II along y, the Poisson solve written into the LDS rows
@@ -315,11 +323,11 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
     if (live) {
         const f32x2 fx = sx[kx];
         const float su = kx == 0 ? 0.5f * s0 : s0;
-        fft_phase2<N1, N2, kFamDct2>(ex, r, t, 0, dtw, [&](int, int, int k, float v) {
-            const f32x2 fy = sy[k];
+        fft_phase2<N1, N2, kFamDct2>(ex, r, t, 0, dtw, [&](int, int, int k, float v, f32x2 c) {
+            const float lv = 4.f * (c.y * c.y);
             const float sc = k == 0 ? 0.5f * su : su;
-            const float den = fx.y + fy.y;
-            prow[k] = den < 1e-30f ? 0.f : v * sc / den;
+            const float den = fx.y + lv;
+            prow[k] = den < 1e-30f ? 0.f : v * sc * __builtin_amdgcn_rcpf(den);
         });
     }
     __syncthreads();
@@ -333,13 +341,15 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
     }
     __syncthreads();
     if (live)
-        fft_phase2<N1, N2, kFamC3>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v) { oa.v[q * N2 + k2] = v; });
+        fft_phase2<N1, N2, kFamC3>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v, f32x2) { oa.v[q * N2 + k2] = v; });
     __syncthreads();
     // Ey path: the sine kind along y of phi w_v
     const int kind = J.kind;
     if (live) {
         f32x2 a[N1];
-        fft_input<N1, N2, kFamSine>(kind, t, dtw, [&](int i) { return prow[i] * sy[i].x; }, a);
+        constexpr double kPi = 3.14159265358979323846;
+        fft_input<N1, N2, kFamSine>(kind, t, dtw, [&](int i) { return prow[i] * (float)(kPi * (double)i / n); },
+                                    a);
         fft_phase1<N1, N2>(a, tw, ex, r, t);
     }
     __syncthreads();  // the Poisson rows are read: they now stage the Ex output
@@ -347,7 +357,7 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
     if (live) fft_stage<N1, N2, kFamC3>(stA, r, t, oa);
     FftOut<N1, N2> ob;
     if (live)
-        fft_phase2<N1, N2, kFamSine>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v) { ob.v[q * N2 + k2] = v; });
+        fft_phase2<N1, N2, kFamSine>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v, f32x2) { ob.v[q * N2 + k2] = v; });
     __syncthreads();  // the exchange is read; its LDS now stages the Ey output
     float* stB = (float*)ex;
     if (live) fft_stage<N1, N2, kFamSine>(stB, r, t, ob);
@@ -385,12 +395,12 @@ __device__ inline void fft_pair(const FftJob& J, int r0, f32x2* ex) {
     __syncthreads();
     FftOut<N1, N2> oa, ob;
     if (live)
-        fft_phase2<N1, N2, kFamSine>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v) { oa.v[q * N2 + k2] = v; });
+        fft_phase2<N1, N2, kFamSine>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v, f32x2) { oa.v[q * N2 + k2] = v; });
     __syncthreads();
     if (live) fft_phase1<N1, N2>(b, tw, ex, r, t);
     __syncthreads();
     if (live)
-        fft_phase2<N1, N2, kFamC3>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v) { ob.v[q * N2 + k2] = v; });
+        fft_phase2<N1, N2, kFamC3>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v, f32x2) { ob.v[q * N2 + k2] = v; });
     __syncthreads();  // the exchange is read; its LDS now stages (Ex, Ey) pairs
     f32x2* stg = ex;
     if (live) {

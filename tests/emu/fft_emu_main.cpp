@@ -28,6 +28,7 @@ static Dim blockIdx;
 static std::barrier<>* g_bar;
 #define __syncthreads() g_bar->arrive_and_wait()
 #define __clz(v) __builtin_clz(v)
+#define __builtin_amdgcn_rcpf(x) (1.0f / (x))  // v_rcp_f32 (1 ulp) on the GPU
 struct DevState { int done; };
 struct DesignDev { int Bx, By; const void* spec_x; const void* spec_y; };
 struct FftJob {
