@@ -146,9 +146,19 @@ __device__ inline double sched(const DevCfg& c, double lam) {
 // the reference's mean-centred formula, wa.rs:88-99):
 //   d+ = e/S (1 + a (u - U/S)),  u = x - x_max, e = exp(a u)
 //   d- = f/Q (1 - a (v - V/Q)),  v = x - x_min, f = exp(-a v)
+// The FAST path takes 1/S and 1/Q by the hardware reciprocal (1 ulp; S and Q
+// are at least 1 for a net with a pin): a correctly rounded division per sum
+// cost 7 % of the WA kernel.
 template <typename T>
 __device__ inline T wa_pin(T a, T u, T e, T S, T U, T v, T f, T Q, T V) {
-    T iS = T(1) / S, iQ = T(1) / Q;
+    T iS, iQ;
+    if constexpr (std::is_same<T, float>::value) {
+        iS = __builtin_amdgcn_rcpf(S);
+        iQ = __builtin_amdgcn_rcpf(Q);
+    } else {
+        iS = T(1) / S;
+        iQ = T(1) / Q;
+    }
     return e * iS * (T(1) + a * (u - U * iS)) - f * iQ * (T(1) - a * (v - V * iQ));
 }
 
@@ -1100,8 +1110,15 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
                     }
             }
             if (wt > 1e-10) {
-                gden[0] = fx / wt;
-                gden[1] = fy / wt;
+                if constexpr (std::is_same<T, float>::value) {
+                    // FAST: one hardware reciprocal (1 ulp of FP32) for both axes
+                    const double iw = (double)__builtin_amdgcn_rcpf((float)wt);
+                    gden[0] = fx * iw;
+                    gden[1] = fy * iw;
+                } else {
+                    gden[0] = fx / wt;
+                    gden[1] = fy / wt;
+                }
             }
         } else {
             gden[0] = (double)((const T*)D.g[par][0])[i];
@@ -1127,10 +1144,13 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
             double pc = pw + lam;  // mod.rs:152-156
             pc = pc < 1.0 ? 1.0 : pc;
             const double gwa[2] = {wl.x, wl.y};
+            // FAST: the preconditioner's reciprocal (pc >= 1) once, in FP32 (1 ulp)
+            const double ipc = std::is_same<T, float>::value ? (double)__builtin_amdgcn_rcpf((float)pc) : 0.0;
 #pragma unroll
             for (int ax = 0; ax < 2; ax++) {
                 const double gw = gwa[ax];
-                T g = (T)((gw + lam * gden[ax]) / pc);  // mod.rs:146-149
+                T g = std::is_same<T, float>::value ? (T)((gw + lam * gden[ax]) * ipc)
+                                                    : (T)((gw + lam * gden[ax]) / pc);  // mod.rs:146-149
                 ((T*)D.g[par][ax])[i] = g;
                 if (it > 0) {
                     double dx = MODE == 0 ? dpv[ax] : (double)((const T*)D.dpos[ax])[i];
