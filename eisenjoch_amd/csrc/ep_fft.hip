@@ -128,10 +128,13 @@ enum FftFamily { kFamDct2 = 0, kFamC3 = 1, kFamSine = 2 };
 // reversed row (DST-III, kind 2) or (0, x_{n-1}, ..., x_1) (IDXST, kind 3).
 // row(i) returns element i of the row.  The loads of a family are the same
 // whatever the run-time kind (indices wrapped into the row where the element
-// is zeroed), so no load's address depends on a branch.
+// is zeroed), so no load's address depends on a branch.  The factor
+// e^{-i pi j/2n} is e^{-i pi t/2n} (one load per thread) times
+// e^{-i pi N2 m/2n} (the same address in every lane: a scalar load).
 template <int N1, int N2, int F, typename Row>
 __device__ inline void fft_input(int kind, int t, const EP_GLOBAL f32x2* dtw, Row row, f32x2 (&a)[N1]) {
     constexpr int n = N1 * N2;
+    const f32x2 dt = F == kFamDct2 ? (f32x2){1.f, 0.f} : dtw[t];
 #pragma unroll
     for (int m = 0; m < N1; m++) {
         const int j = t + N2 * m;
@@ -141,7 +144,7 @@ __device__ inline void fft_input(int kind, int t, const EP_GLOBAL f32x2* dtw, Ro
             const int ia = F == kFamC3 ? j : kind == 2 ? n - 1 - j : (n - j) & (n - 1);
             const int ib = F == kFamC3 ? (n - j) & (n - 1) : kind == 2 ? (j - 1) & (n - 1) : j;
             const float xa = row(ia), xb = row(ib);
-            const f32x2 c = dtw[j];
+            const f32x2 c = m == 0 ? dt : cmul(dt, dtw[N2 * m]);
             const float za = (F == kFamSine && kind == 3 && j == 0) ? 0.f : xa;
             const float zb = j == 0 ? 0.f : xb;
             a[m] = cmul(c, (f32x2){za, zb});
@@ -160,6 +163,21 @@ __device__ inline void fft_phase1(f32x2 (&a)[N1], const EP_GLOBAL f32x2* tw, f32
     dft_reg<N1>(a);
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++) ex[(r * N1 + k1) * PAD + t] = cmul(a[k1], tw[k1 * N2 + t]);
+}
+
+// The thread's phase-1 twiddles in registers, for kernels that take several
+// transforms of the same length (loaded once, with the kernel's first loads).
+template <int N1, int N2>
+__device__ inline void load_tw(const EP_GLOBAL f32x2* tw, int t, f32x2 (&w)[N1]) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) w[k1] = tw[k1 * N2 + t];
+}
+template <int N1, int N2>
+__device__ inline void fft_phase1(f32x2 (&a)[N1], const f32x2 (&w)[N1], f32x2* ex, int r, int t) {
+    constexpr int PAD = N2 + 1;
+    dft_reg<N1>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) ex[(r * N1 + k1) * PAD + t] = cmul(a[k1], w[k1]);
 }
 
 // Output index of phase-2 value (q, k2) of thread t: k = k1 + N1 k2 with
@@ -188,12 +206,14 @@ __device__ inline void fft_phase2(const f32x2* ex, int r, int t, int kind, const
             f32x2 b[N2];
 #pragma unroll
             for (int s2 = 0; s2 < N2; s2++) b[s2] = ex[(r * N1 + k1) * PAD + s2];
+            // DCT-II: e^{-i pi k/2n} = e^{-i pi k1/2n} (one load) e^{-i pi N1 k2/2n} (a scalar load)
+            const f32x2 dk = F == kFamDct2 ? dtw[k1] : (f32x2){1.f, 0.f};
             dft_reg<N2>(b);
 #pragma unroll
             for (int k2 = 0; k2 < N2; k2++) {
                 const int j = fft_out_index<N1, N2, F>(t, q, k2);
                 if (F == kFamDct2) {
-                    const f32x2 c = dtw[j];
+                    const f32x2 c = k2 == 0 ? dk : cmul(dk, dtw[N1 * k2]);
                     emit(q, k2, j, c.x * b[k2].x - c.y * b[k2].y, c);
                 } else {
                     const float y = 0.5f * b[k2].x;
@@ -313,11 +333,13 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
     const EP_GLOBAL f32x2* sx = (const EP_GLOBAL f32x2*)D.spec_x;
     const float s0 = 4.0f / ((float)D.Bx * (float)D.By);
     float* prow = phi + r * (n + 1);
+    f32x2 twr[N1];  // the three transforms' phase-1 twiddles
     // forward DCT-II along y, the Poisson solve written into the LDS rows
     if (live) {
         f32x2 a[N1];
+        load_tw<N1, N2>(tw, t, twr);
         fft_input<N1, N2, kFamDct2>(0, t, dtw, [&](int i) { return x[i]; }, a);
-        fft_phase1<N1, N2>(a, tw, ex, r, t);
+        fft_phase1<N1, N2>(a, twr, ex, r, t);
     }
     __syncthreads();
     if (live) {
@@ -337,7 +359,7 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
         const float wu = sx[kx].x;
         f32x2 a[N1];
         fft_input<N1, N2, kFamC3>(1, t, dtw, [&](int i) { return prow[i] * wu; }, a);
-        fft_phase1<N1, N2>(a, tw, ex, r, t);
+        fft_phase1<N1, N2>(a, twr, ex, r, t);
     }
     __syncthreads();
     if (live)
@@ -350,7 +372,7 @@ __device__ inline void fft_solve(const FftJob& J, const DesignDev* dd, int r0, f
         constexpr double kPi = 3.14159265358979323846;
         fft_input<N1, N2, kFamSine>(kind, t, dtw, [&](int i) { return prow[i] * (float)(kPi * (double)i / n); },
                                     a);
-        fft_phase1<N1, N2>(a, tw, ex, r, t);
+        fft_phase1<N1, N2>(a, twr, ex, r, t);
     }
     __syncthreads();  // the Poisson rows are read: they now stage the Ex output
     float* stA = phi;
@@ -385,19 +407,20 @@ __device__ inline void fft_pair(const FftJob& J, int r0, f32x2* ex) {
     const EP_GLOBAL float* xb = (const EP_GLOBAL float*)J.in2 + (size_t)(r0 + r) * n;
     const EP_GLOBAL f32x2* tw = (const EP_GLOBAL f32x2*)J.tw;
     const EP_GLOBAL f32x2* dtw = (const EP_GLOBAL f32x2*)J.dtw;
-    f32x2 b[N1];
+    f32x2 b[N1], twr[N1];  // twr: both transforms' phase-1 twiddles
     if (live) {
         f32x2 a[N1];
+        load_tw<N1, N2>(tw, t, twr);
         fft_input<N1, N2, kFamSine>(kind, t, dtw, [&](int i) { return xa[i]; }, a);
         fft_input<N1, N2, kFamC3>(1, t, dtw, [&](int i) { return xb[i]; }, b);
-        fft_phase1<N1, N2>(a, tw, ex, r, t);
+        fft_phase1<N1, N2>(a, twr, ex, r, t);
     }
     __syncthreads();
     FftOut<N1, N2> oa, ob;
     if (live)
         fft_phase2<N1, N2, kFamSine>(ex, r, t, kind, dtw, [&](int q, int k2, int, float v, f32x2) { oa.v[q * N2 + k2] = v; });
     __syncthreads();
-    if (live) fft_phase1<N1, N2>(b, tw, ex, r, t);
+    if (live) fft_phase1<N1, N2>(b, twr, ex, r, t);
     __syncthreads();
     if (live)
         fft_phase2<N1, N2, kFamC3>(ex, r, t, 1, dtw, [&](int q, int k2, int, float v, f32x2) { ob.v[q * N2 + k2] = v; });
