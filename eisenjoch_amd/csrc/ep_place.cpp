@@ -695,10 +695,9 @@ void fill_result(const Design& D, const DevState& s, ep_result* r) {
 }
 
 // Runs a built batch until every design stops.  The next chunk of
-// iterations is always queued before the host waits for the states copied
-// behind the current one, so the GPU does not idle while the host checks
-// them; a finished batch runs one chunk of iterations that skip every
-// design.  Each design's time is taken on the device, from the reset to the
+// iterations is queued before the host waits for the states copied behind
+// the current one, so the GPU does not idle while the host checks them,
+// except where the states say the batch is about to finish.  Each design's time is taken on the device, from the reset to the
 // control step that stopped it, so it does not depend on when the host looks.
 int batch_place(Ctx* c, Batch& b, int sync_every, bool use_graph) {
     int rc = batch_reset(c, b, nullptr);
@@ -723,21 +722,44 @@ int batch_place(Ctx* c, Batch& b, int sync_every, bool use_graph) {
         return EP_OK;
     };
     int queued = 0;
+    // Once every design has stopped, the chunk queued behind the one whose
+    // states the host reads runs iterations that skip every design, and each
+    // still costs a launch per kernel (about 35 us on the 64-design batch).
+    // When every live design is past its minimum iteration count within the
+    // next chunk and within 5 % of its overflow target, that chunk will most
+    // likely stop them all, so nothing is queued behind it; a wrong guess
+    // costs one host round trip with the GPU idle.
+    auto ending = [&]() {
+        for (size_t k = 0; k < b.ds.size(); k++) {
+            const DevState& s = c->state_host[b.ds[k]->state_idx];
+            if (s.done) continue;
+            const ep_config& cf = b.cfgs[k];
+            if (s.it + sync_every - 1 < cf.min_iters || !(s.ov < cf.overflow_target * 1.05)) return false;
+        }
+        return true;
+    };
     // a penalty once set stays set, so a mirror one chunk old can only ask
     // for the fix pass where it is no longer needed (it then does nothing)
     bool fix = batch_needs_fix(c, b);
+    bool ahead = true;
     if (!all_done() && max_it > 0) {
         if ((rc = batch_run_iters(c, b, sync_every, use_graph, fix))) return rc;
         queued += sync_every;
         if ((rc = copy_states())) return rc;
         for (;;) {
-            if (queued < max_it) {
+            const bool queued_ahead = ahead && queued < max_it;
+            if (queued_ahead) {
                 if ((rc = batch_run_iters(c, b, sync_every, use_graph, fix))) return rc;
                 queued += sync_every;
             }
             HIPCHK(c, hipEventSynchronize(c->ev_states));
             if (all_done() || queued >= max_it) break;
             fix = batch_needs_fix(c, b);
+            ahead = !ending();
+            if (!queued_ahead) {  // the guess was wrong: the next chunk now
+                if ((rc = batch_run_iters(c, b, sync_every, use_graph, fix))) return rc;
+                queued += sync_every;
+            }
             if ((rc = copy_states())) return rc;
         }
     }
