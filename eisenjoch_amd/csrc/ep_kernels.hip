@@ -495,14 +495,20 @@ __device__ inline double acc_value(unsigned long long whole, unsigned long long 
     else return (double)whole + (double)frac * 5.421010862427522e-20;  // 2^-64
 }
 
-// density.rs:41-48
-__device__ inline double ov_area(const BinBox& r, int bx, int by) {
-    double ox = fmin(r.xh, (double)bx + 1.0) - fmax(r.xl, (double)bx);
-    double oy = fmin(r.yh, (double)by + 1.0) - fmax(r.yl, (double)by);
-    ox = ox < 0.0 ? 0.0 : ox;
-    oy = oy < 0.0 ? 0.0 : oy;
-    return ox * oy;
+// density.rs:41-48, as the product of the overlap along x with bin column
+// bx and along y with bin row by: the gather forms a cell's three column
+// widths and row heights once and multiplies them per bin (the same bits as
+// ov_area).  The pull scatter keeps ov_area per bin: there the per-axis form
+// measured slower.
+__device__ inline double ov_x(const BinBox& r, int bx) {
+    const double o = fmin(r.xh, (double)bx + 1.0) - fmax(r.xl, (double)bx);
+    return o < 0.0 ? 0.0 : o;
 }
+__device__ inline double ov_y(const BinBox& r, int by) {
+    const double o = fmin(r.yh, (double)by + 1.0) - fmax(r.yl, (double)by);
+    return o < 0.0 ? 0.0 : o;
+}
+__device__ inline double ov_area(const BinBox& r, int bx, int by) { return ov_x(r, bx) * ov_y(r, by); }
 
 // Scatter for designs whose widest footprint is too wide for the pull
 // scatter below.  One block covers kScatCells consecutive cells of this
@@ -1085,11 +1091,17 @@ __global__ __launch_bounds__(256) void k_gather(BatchDev b, double* ex_out, doub
                         e1[q] = in ? e.y : T(0);
                     }
                 }
+                double oxs[3], oys[3];  // w = ov_area(r, bx, by), formed per axis
+#pragma unroll
+                for (int u = 0; u < 3; u++) {
+                    oxs[u] = ov_x(r, r.bx0 + u);
+                    oys[u] = ov_y(r, r.by0 + u);
+                }
 #pragma unroll
                 for (int q = 0; q < 9; q++) {
                     const int by = r.by0 + q / 3, bx = r.bx0 + q % 3;
                     if (by < r.by1 && bx < r.bx1) {
-                        double w = ov_area(r, bx, by);
+                        double w = oxs[q % 3] * oys[q / 3];
                         if (w > 0.0) {
                             fx += w * (double)e0[q];
                             fy += w * (double)e1[q];
