@@ -19,7 +19,7 @@ constexpr int kMaxBuckets = 6;       // group sizes 2,4,8,16,32,64
 constexpr uint32_t kPad = 0xFFFFFFFFu;  // empty pin slot
 constexpr int kCellParts = 8;        // partial sums per cell block
 constexpr int kBinParts = 2;         // partial sums per bin block
-constexpr double kFixScale = 4294967296.0;  // 2^32 fixed point for deterministic sums
+constexpr double kFixScale = 2147483648.0;  // 2^31 fixed point for deterministic sums (contributions <= 1)
 constexpr int kScatCells = 1024;     // cells per scatter block
 constexpr int kScatBins = 6144;      // LDS accumulation window of a scatter block (bins)
 constexpr int kHfStat = 16;      // per high-fanout net: 4 extremes, 8 sums, weight, pad
@@ -120,7 +120,7 @@ struct DesignDev {
     uint32_t* hf_off;          // [n_hf + 1]  (nets with > 64 pins)
     uint32_t* hf_pins;
     // bins
-    void* dacc;                // wide-cell scatter accumulator: FAST u64 fixed point 2^-32 [By*Bx];
+    void* dacc;                // wide-cell scatter accumulator: FAST u64 fixed point 2^-31 [By*Bx];
                                // FP64 exact 128-bit fixed point, integer words [By*Bx] then
                                // 2^-64 fraction words [By*Bx]
     void* dens;                // T [By*Bx]

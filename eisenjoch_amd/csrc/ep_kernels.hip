@@ -30,7 +30,7 @@
 //   k_update      Nesterov step + clamp + next look-ahead (nesterov.rs:41-72,
 //                 mod.rs:105-107,169-173).
 //
-// Every FAST-path sum that crosses threads is integer (2^32 fixed point) or
+// Every FAST-path sum that crosses threads is integer (2^31 fixed point) or
 // in a fixed order, so a design's trajectory is the same alone and inside
 // any batch.
 #include <hip/hip_runtime.h>
@@ -472,15 +472,18 @@ __device__ inline BinBox bin_box(const DesignDev& D, double cx, double cy, doubl
     return r;
 }
 
-// Density accumulators.  FAST: 64-bit fixed point 2^-32 (one word per bin).
-// FP64: exact 128-bit fixed point, an integer word and a 2^-64 fraction word
-// per bin (FIX = false); the fraction's carry goes to the integer word, so the
-// sum is the exact sum of the truncated contributions in any order and the
-// plain placer's density is deterministic to the last bit.
+// Density accumulators.  FAST: 64-bit fixed point 2^-31 (one word per bin);
+// a contribution is one bin's overlap, at most 1, so it converts through 32
+// bits (one rounding and one conversion instruction where a 64-bit
+// conversion takes a sequence).  FP64: exact 128-bit fixed point, an integer
+// word and a 2^-64 fraction word per bin (FIX = false); the fraction's carry
+// goes to the integer word, so the sum is the exact sum of the truncated
+// contributions in any order and the plain placer's density is deterministic
+// to the last bit.
 template <bool FIX>
 __device__ inline void acc_add(unsigned long long* whole, unsigned long long* frac, double w) {
     if constexpr (FIX) {
-        atomicAdd(whole, (unsigned long long)__double2ll_rn(w * kFixScale));
+        atomicAdd(whole, (unsigned long long)(uint32_t)rint(w * kFixScale));
     } else {
         const double ip = floor(w);
         const unsigned long long f = __double2ull_rz((w - ip) * 18446744073709551616.0);

@@ -4,7 +4,7 @@ Tolerances, per kernel and precision (relative to the largest magnitude of
 the oracle's output unless stated):
   WA gradient: FP64 1e-12, FAST 2e-5 (FP32 exponentials and sums);
   density: FP64 1e-13 absolute, FAST 2e-7 relative + 1e-8 absolute
-    (2^-32 fixed point); overflow FP64 1e-12, FAST 1e-7 relative;
+    (2^-31 fixed point); overflow FP64 1e-12, FAST 1e-7 relative;
   field (from the oracle's density): FP64 1e-10, FAST 1e-4 (BF16 hi+lo MFMA);
   density gradient (from the oracle's field): FP64 1e-13, FAST 1e-6;
   placement: FAST and plain FP64 both reach overflow <= 0.10 and their

@@ -74,22 +74,33 @@ def test_fft_engine_single_modes(device):
 
 def test_fft_engine_placement_matches_mfma(device):
     # one 256-bin design placed on the FFT engine and on engine 0: both reach
-    # overflow <= 0.10, with HPWL within 0.1 % of each other
+    # overflow <= 0.10 in as many iterations as the plain FP64 placer, with
+    # HPWL within 0.25 % of its (half the 0.5 % bar of the FAST path; the two
+    # FAST runs end 0.01-0.1 % from it, differences in rounding moving the
+    # end point of the trajectory by that much) and of each other
     d = ep.sized_design(40000, 33)
     assert d.Bx == 256
     cfg = ep.ElectroPlaceCfg(max_iters=500, field_mode="idxst", log_hpwl=False)
+    i64 = device.upload(d, "fp64")
     i = device.upload(d, "fast")
     try:
+        log_case("plain FP64 placement at 256 bins")
+        r64 = device.place(i64, cfg, positions=False)
         out = {}
         for e in (0, FFT):
             log_case(f"placement with engine {e} at 256 bins")
             device.tune_set(256, 256, e)
             out[e] = device.place(i, cfg, positions=False)
-        for r in out.values():
+        for e, r in out.items():
             assert r.converged and r.overflow <= 0.10
-        assert abs(out[FFT].hpwl - out[0].hpwl) / out[0].hpwl < 1e-3
+            assert abs(r.iters - r64.iters) <= 2
+            dev = abs(r.hpwl - r64.hpwl) / r64.hpwl
+            print(f"  engine {e}: HPWL {dev:.2e} from FP64", flush=True)
+            assert dev < 2.5e-3
+        assert abs(out[FFT].hpwl - out[0].hpwl) / out[0].hpwl < 2.5e-3
     finally:
         device.free(i)
+        device.free(i64)
         device.tune_set(256, 256, -1)
 
 

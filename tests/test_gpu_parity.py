@@ -123,7 +123,7 @@ def test_density_and_overflow_parity(device, designs, prec, name):
             if prec == "fp64":
                 np.testing.assert_allclose(dens, od, rtol=1e-13, atol=1e-13)
                 assert ov == pytest.approx(oov, rel=1e-12)
-            else:  # fixed-point 2^-32 sums, stored as float
+            else:  # fixed-point 2^-31 sums, stored as float
                 np.testing.assert_allclose(dens, od, rtol=2e-7, atol=1e-8)
                 assert ov == pytest.approx(oov, rel=1e-8, abs=1e-10)
             # invariant: total = cell area inside the grid
