@@ -105,6 +105,8 @@ struct DesignDev {
     void* hf_stat;             // T [n_hf][kHfStat]: extremes, exponential sums, weight
     uint32_t* perm[2];         // cells in Morton order of their lower-left bin tile; the
                                // sort of iteration k reads perm[k&1] and writes perm[(k+1)&1]
+    uint32_t* perm0;           // [n_mov] the order a reset starts from: the cells by the tile of
+                               // their initial position, so the first sort's waves see few keys
     uint32_t* keys;            // tile key of perm[k&1][j], by position j
     void* stage_xy;            // T [n_mov][2]: look-ahead position of perm[k&1][j], by j
     float* stage_hs;           // half side of perm[k&1][j], by j

@@ -1422,8 +1422,9 @@ __global__ __launch_bounds__(256) void k_reset(BatchDev b) {
     const int i = ref.blk * kBlock + threadIdx.x;
     if (i >= D.n_mov) return;
     if (i == 0) D.st->t_start = wall_clock64();  // the placement's start, for its latency
-    D.perm[0][i] = (uint32_t)i;
-    D.sorted_hs[0][i] = D.hs[i];
+    const uint32_t c0 = D.perm0[i];
+    D.perm[0][i] = c0;
+    D.sorted_hs[0][i] = D.hs[c0];
     T lk[2];
 #pragma unroll
     for (int ax = 0; ax < 2; ax++) {

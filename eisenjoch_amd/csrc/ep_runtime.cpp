@@ -316,6 +316,7 @@ static size_t plan_arena(Design& D, char* base) {
     G.hf_stat = P((size_t)G.n_hf * kHfStat * T);
     G.perm[0] = (uint32_t*)P(N * 4);
     G.perm[1] = (uint32_t*)P(N * 4);
+    G.perm0 = (uint32_t*)P(N * 4);
     G.keys = (uint32_t*)P(N * 4);
     G.stage_xy = P(N * 2 * T);
     G.stage_hs = (float*)P(N * 4);
@@ -617,9 +618,27 @@ int prepare_design(Ctx* c, const ep_design* d, int prec, int log_cap, Prepared& 
                 sw[base + p] = d->net_weight[P.L.hf_net[h]];
         to_prec(P.slot_w, sw.data(), sw.size(), D.prec);
     }
-    // scatter order: any permutation of the cells is valid; start with the identity
+    // scatter order: any permutation of the cells is valid.  A placement starts
+    // from the cells ordered by the (row-major) tile of their initial
+    // position, so that the first sort, like every later one, walks cells
+    // that mostly share a tile; in index order its waves saw dozens of keys
     P.ident.resize(N);
     for (size_t i = 0; i < N; i++) P.ident[i] = (uint32_t)i;
+    {
+        std::vector<uint64_t> key(N);
+        for (size_t i = 0; i < N; i++) {
+            const int tx = std::min(std::max((int)std::floor(d->x0[i] * D.dev.rx), 0), D.dev.Bx - 1) >> 3;
+            const int ty = std::min(std::max((int)std::floor(d->y0[i] * D.dev.ry), 0), D.dev.By - 1) >> 3;
+            key[i] = ((uint64_t)((uint32_t)ty * (uint32_t)D.dev.tiles_x + (uint32_t)tx) << 32) | i;
+        }
+        std::sort(key.begin(), key.end());
+        P.order.resize(N);
+        P.order_hs.resize(N);
+        for (size_t j = 0; j < N; j++) {
+            P.order[j] = (uint32_t)(key[j] & 0xffffffffu);
+            P.order_hs[j] = P.hs[P.order[j]];
+        }
+    }
     const double pi = 3.14159265358979323846;
     for (int ax = 0; ax < 2; ax++) {  // density.rs:135-140
         const int B = ax == 0 ? D.dev.Bx : D.dev.By;
@@ -701,9 +720,10 @@ int commit_design(Ctx* c, Prepared& P, uint32_t* id_out) {
         rc = copy_in(c, D.dev.x0[ax], P.x0[ax].data(), P.x0[ax].size());
         if (!rc) rc = copy_in(c, (char*)D.dev.pos[ax] + N * Tb, P.fixed[ax].data(), P.fixed[ax].size());
         if (!rc) rc = copy_in(c, ax == 0 ? D.dev.spec_x : D.dev.spec_y, P.spec[ax].data(), P.spec[ax].size());
-        if (!rc) rc = copy_in(c, D.dev.perm[ax], P.ident.data(), N * 4);
-        if (!rc) rc = copy_in(c, D.dev.sorted_hs[ax], P.hs.data(), N * 4);
+        if (!rc) rc = copy_in(c, D.dev.perm[ax], ax == 0 ? P.order.data() : P.ident.data(), N * 4);
+        if (!rc) rc = copy_in(c, D.dev.sorted_hs[ax], ax == 0 ? P.order_hs.data() : P.hs.data(), N * 4);
     }
+    if (!rc) rc = copy_in(c, D.dev.perm0, P.order.data(), N * 4);
     if (!rc) rc = copy_in(c, D.dev.pinw, P.pinw.data(), P.pinw.size());
     if (!rc) rc = copy_in(c, D.dev.hs, P.hs.data(), N * 4);
     if (!rc) rc = copy_in(c, D.dev.slots, L.slots.data(), L.slots.size() * 4);

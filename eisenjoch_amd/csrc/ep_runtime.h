@@ -164,6 +164,8 @@ struct Prepared {
     std::vector<char> slot_w;   // T per slot: its net's weight (empty: all nets weigh 1)
     std::vector<float> hs;
     std::vector<uint32_t> ident;
+    std::vector<uint32_t> order;  // the cells by the tile of their initial position
+    std::vector<float> order_hs;  // their half sides, in that order
 };
 
 // Validates d and fills P (host only; c receives the error, may be null).
