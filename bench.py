@@ -1,8 +1,8 @@
 """Benchmark of the batched electrostatic placer on one MI355X.
 
 Workload (the batched configuration): 64 independent synthetic designs with
-log-uniform sizes of 2,000-100,000 movable cells (seed 0; design i uses seed
-i), nets = 1.1 x cells, grid side = round(sqrt(cells / 0.7)), bins = nearest
+log-uniform sizes of 2,000-100,000 movable cells (seed 0 unless --seed S;
+design i uses seed S + i), nets = 1.1 x cells, grid side = round(sqrt(cells / 0.7)), bins = nearest
 power of two to the side in [64, 512], cell areas 1/2/4.  One step places the
 whole batch from its initial positions until every design reaches overflow
 <= 0.10 (from iteration 20 on; checked every iteration by default, every 5
@@ -44,6 +44,8 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--designs", type=int, default=64)
+    p.add_argument("--seed", type=int, default=0,
+                   help="batch: seed of the sizes; design i uses seed + i")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU oracle sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
@@ -418,8 +420,7 @@ def main():
         sys.exit(2)
     import eisenjoch_amd as ep
 
-    sizes = ep.batch_sizes(args.designs)
-    designs = [ep.sized_design(n, i) for i, n in enumerate(sizes)]
+    designs = ep.batch_designs(args.designs, seed=args.seed)
     cfg = place_cfg(args)
     dev = ep.Device(0)
     ids = [dev.upload(d, "fast") for d in designs]
@@ -476,6 +477,7 @@ def main():
                         "pow2 bins 64-512, areas 1/2/4, idxst field, stop overflow<=0.10",
             "stop_check": "every iteration" if args.stop == "every" else "every 5 iterations",
             "designs": len(ids),
+            "seed": args.seed,
             "total_cells": int(sum(d.n_mov for d in designs)),
             "total_pins": int(sum(d.n_pins for d in designs)),
             "converged": int(conv),
